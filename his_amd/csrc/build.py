@@ -1,0 +1,60 @@
+"""Compile the gfx950 HIP sources into his_amd/_lib/libhis_amd.so.
+
+    python -m his_amd.csrc.build        (or his_amd.csrc.build.build())
+
+float64 translation units are compiled without floating-point contraction so that
+each operation rounds once, as in the reference's eager float64 path; the float32
+bulk kernels allow contraction and the faster float32 divide and square root.
+Packed-FP32 instructions are disabled for every unit.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+PKG = HERE.parent
+OUT = PKG / "_lib" / "libhis_amd.so"
+BUILD = PKG / "_lib" / "_build"
+ARCH = os.environ.get("HIS_OFFLOAD_ARCH", "gfx950")
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I", str(HERE)]
+UNITS = {
+    "swe2d_f64.hip": ["-ffp-contract=off"],
+    "swe2d_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "net1d.hip": ["-ffp-contract=off"],
+    "engine.hip": ["-ffp-contract=off"],
+}
+
+
+def _compile(src: str, flags: list, verbose: bool) -> Path:
+    obj = BUILD / (Path(src).stem + ".o")
+    cmd = ["hipcc", "-c", str(HERE / src), "-o", str(obj)] + COMMON + flags
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-6000:]}")
+    return obj
+
+
+def build(verbose: bool = False, jobs: int = 4) -> Path:
+    BUILD.mkdir(parents=True, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], verbose), UNITS.items()))
+    tmp = str(OUT) + ".tmp"
+    cmd = ["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + [str(o) for o in objs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    Path(tmp).replace(OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

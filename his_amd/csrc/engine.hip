@@ -1,0 +1,831 @@
+// C ABI entry points and the device-resident coupled engine.
+//
+// The coupled step keeps the order of CoupledEngine.step (hydrointel/solver/coupling.py:201-227):
+//   dt from the pre-exchange state (2-D CFL, clamped; never past the output time),
+//   n_sub from the 1-D CFL, the weir exchange, the 2-D step with rain(t) and edge values at t and
+//   t+dt, then n_sub 1-D substeps at t + k*dt/n_sub.
+// Nothing crosses to the host: the CFL rates are reduced in the producing kernels' epilogues, the
+// forcing is evaluated from device tables, and the float64 ledger lives on the device.  One step
+// (or several) is captured into a hipGraph; the 1-D network branch runs beside the 2-D stages, since
+// after the exchange the two models do not read each other until the next step.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "net1d.hpp"
+#include "swe2d_desc.hpp"
+
+namespace his {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    set_error("%s: %s", where, hipGetErrorString(e));
+    return HIS_E_HIP;
+}
+
+int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d* N, const double* A, double* out,
+                 hipStream_t s);
+
+// ------------------------------------------------------------------ finalize
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
+    __shared__ double red[NT / 64][24];
+    const int m = blockIdx.x;
+    if (F.active && !F.active[m]) return;
+    int gi = 0;
+    while (gi + 1 < F.ngroups && m >= F.member0[gi + 1]) ++gi;
+    const int b = m - F.member0[gi];
+    const int per = F.per[gi];
+    const int64_t t0 = F.tile0[gi] + (int64_t)b * per;
+    double s1[P_NOUT + 1], s2[P_USED - 1];
+    for (int k = 0; k <= P_NOUT; ++k) s1[k] = 0.0;
+    for (int k = 0; k < P_USED - 1; ++k) s2[k] = 0.0;
+    double rmax = 0.0;
+    for (int t = threadIdx.x; t < per; t += NT) {
+        const double* p = F.part + (t0 + t) * NP2;
+        if (!F.stage_only)
+            for (int k = 0; k <= P_NOUT; ++k) s1[k] += p[k];
+        for (int k = 0; k < P_USED - 1; ++k) s2[k] += p[NP + k];
+        rmax = p[NP + P_RATE] > rmax ? p[NP + P_RATE] : rmax;
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k <= P_NOUT; ++k) s1[k] = wave_sum(s1[k]);
+    for (int k = 0; k < P_USED - 1; ++k) s2[k] = wave_sum(s2[k]);
+    rmax = wave_max(rmax);
+    if (lane == 0) {
+        for (int k = 0; k <= P_NOUT; ++k) red[w][k] = s1[k];
+        for (int k = 0; k < P_USED - 1; ++k) red[w][10 + k] = s2[k];
+        red[w][23] = rmax;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k <= P_NOUT; ++k) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < NT / 64; ++w2) s += red[w2][k];
+        s1[k] = s;
+    }
+    for (int k = 0; k < P_USED - 1; ++k) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < NT / 64; ++w2) s += red[w2][10 + k];
+        s2[k] = s;
+    }
+    rmax = 0.0;
+    for (int w2 = 0; w2 < NT / 64; ++w2) rmax = red[w2][23] > rmax ? red[w2][23] : rmax;
+    const double area = F.area[gi];
+    if (F.rate) F.rate[m] = rmax;
+    if (F.volume) {
+        F.volume[m] = s2[P_VOLH] * area;
+        F.volume[F.Btot + m] = s2[P_VOLS] * area;
+    }
+    if (F.stage_only) return;
+    const double dt = F.dt[m], dx = F.dx[gi], dy = F.dy[gi];
+    const double q2i = (s2[P_WIN] + s2[P_EIN]) * dy + (s2[P_SIN] + s2[P_NIN]) * dx;
+    const double q2o = (s2[P_WOUT] + s2[P_EOUT]) * dy + (s2[P_SOUT] + s2[P_NOUT]) * dx;
+    double qin, qout, clip;
+    if (F.order[gi] == 2) {
+        const double q1i = (s1[P_WIN] + s1[P_EIN]) * dy + (s1[P_SIN] + s1[P_NIN]) * dx;
+        const double q1o = (s1[P_WOUT] + s1[P_EOUT]) * dy + (s1[P_SOUT] + s1[P_NOUT]) * dx;
+        qin = 0.5 * (q1i + q2i);
+        qout = 0.5 * (q1o + q2o);
+        clip = s1[P_CLIP] * area + s2[P_CLIP] * area;
+    } else {
+        qin = q2i;
+        qout = q2o;
+        clip = s2[P_CLIP] * area;
+    }
+    const double rain = F.pm[m];
+    double rain_v;
+    if (F.weighted[gi]) rain_v = s2[P_RAINW] * area;
+    else {
+        const double rd = F.rain_f32[gi] ? (double)((float)rain * (float)dt) : rain * dt;
+        rain_v = rd * (F.cells[gi] * area);
+    }
+    const double inf_v = s2[P_INF] * area;
+    const double bin = qin * dt, bout = qout * dt;
+    if (F.vols) {
+        F.vols[m] = bin;
+        F.vols[F.Btot + m] = bout;
+        F.vols[2 * F.Btot + m] = rain_v;
+        F.vols[3 * F.Btot + m] = inf_v;
+        F.vols[4 * F.Btot + m] = clip;
+    }
+    if (F.acc) {
+        F.acc[0 * F.Btot + m] += rain_v;
+        F.acc[1 * F.Btot + m] += inf_v;
+        F.acc[2 * F.Btot + m] += bin;
+        F.acc[3 * F.Btot + m] += bout;
+        F.acc[6 * F.Btot + m] += clip;
+    }
+}
+
+int launch_finalize(const FinalizeArgs& f, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize<256>, dim3(f.Btot), dim3(256), 0, s, f);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ building launches
+
+static int g_shape[2] = {0, 0};   // tile-shape index per dtype (HIS_F64, HIS_F32)
+
+static int shape_of(int dtype, int* tx, int* ty) {
+    const TileShape& t = dtype == HIS_F64 ? kShapesF64[g_shape[0]] : kShapesF32[g_shape[1]];
+    *tx = t.tx;
+    *ty = t.ty;
+    return dtype == HIS_F64 ? g_shape[0] : g_shape[1];
+}
+
+static int check_grid(const his_grid2d* g) {
+    if (!g) { set_error("grid is NULL"); return HIS_E_ARG; }
+    if (g->dtype != HIS_F64 && g->dtype != HIS_F32) { set_error("dtype must be HIS_F64 or HIS_F32"); return HIS_E_ARG; }
+    if (g->B < 1 || g->ny < 1 || g->nx < 1) { set_error("bad grid size B=%d ny=%d nx=%d", g->B, g->ny, g->nx); return HIS_E_ARG; }
+    if (g->order != 1 && g->order != 2) { set_error("order must be 1 or 2"); return HIS_E_ARG; }
+    for (int e = 0; e < 4; ++e)
+        if (g->bc[e] < 0 || g->bc[e] > 3) { set_error("bad edge kind %d", g->bc[e]); return HIS_E_ARG; }
+    if (!g->z || !g->manning) { set_error("z and manning are required"); return HIS_E_ARG; }
+    if (g->f0 && (!g->fc || !g->k)) { set_error("Horton needs f0, fc and k"); return HIS_E_ARG; }
+    return 0;
+}
+
+template <typename R>
+static void fill_group(Group2D<R>& G, const his_grid2d& g, const his_state2d& st, R* tmp, int member0) {
+    G.B = g.B; G.ny = g.ny; G.nx = g.nx; G.order = g.order; G.flux = g.flux;
+    for (int e = 0; e < 4; ++e) G.bc[e] = g.bc[e];
+    G.member0 = member0;
+    G.dx = (R)g.dx; G.dy = (R)g.dy; G.g = (R)g.g; G.h_dry = (R)g.h_dry; G.kp_eps = (R)g.kp_eps;
+    G.dxd = g.dx; G.dyd = g.dy; G.gd = g.g; G.t0 = g.horton_t0; G.area = g.dx * g.dy;
+    G.z = (const R*)g.z; G.n = (const R*)g.manning; G.S = (const R*)g.storage;
+    G.f0 = (const R*)g.f0; G.fc = (const R*)g.fc; G.k = (const R*)g.k; G.rw = (const R*)g.rain_weight;
+    G.n_bs = g.manning_bs; G.S_bs = g.storage_bs; G.hz_bs = g.horton_bs; G.rw_bs = g.rain_weight_bs;
+    G.h = (R*)st.h; G.hu = (R*)st.hu; G.hv = (R*)st.hv; G.s = (R*)st.s_filled; G.ic = (R*)st.infil_cum;
+    const int64_t n = (int64_t)g.B * g.ny * g.nx;
+    G.h1 = tmp; G.hu1 = tmp ? tmp + n : nullptr; G.hv1 = tmp ? tmp + 2 * n : nullptr;
+}
+
+static int64_t max_tiles(const his_grid2d& g) {
+    const TileShape* T = g.dtype == HIS_F64 ? kShapesF64 : kShapesF32;
+    int64_t best = 0;
+    for (int k = 0; k < kNumShapes; ++k) {
+        int64_t t = (int64_t)g.B * ((g.nx + T[k].tx - 1) / T[k].tx) * ((g.ny + T[k].ty - 1) / T[k].ty);
+        best = t > best ? t : best;
+    }
+    return best;
+}
+
+static size_t esize(int dtype) { return dtype == HIS_F64 ? 8 : 4; }
+
+static size_t ws2d_bytes(const his_grid2d& g) {
+    const size_t tmp = align_up(3 * (size_t)g.B * g.ny * g.nx * esize(g.dtype));
+    const size_t part = align_up((size_t)max_tiles(g) * NP2 * sizeof(double));
+    return tmp + part;
+}
+
+static void fill_finalize(FinalizeArgs& F, int gi, const his_grid2d& g, int member0, int tile0, int per) {
+    F.member0[gi] = member0; F.B[gi] = g.B; F.tile0[gi] = tile0; F.per[gi] = per;
+    F.order[gi] = g.order; F.weighted[gi] = g.rain_weight != nullptr;
+    F.dx[gi] = g.dx; F.dy[gi] = g.dy; F.area[gi] = g.dx * g.dy; F.cells[gi] = (double)((int64_t)g.nx * g.ny);
+    F.rain_f32[gi] = g.dtype == HIS_F32;
+}
+
+template <typename R>
+static int op_step(const his_grid2d* g, his_state2d st, const double* dt_t, const double* pm, double* vols,
+                   double* rate, void* ws, hipStream_t s) {
+    Launch2D<R> L{};
+    L.ngroups = 1;
+    L.Btot = g->B;
+    R* tmp = (R*)ws;
+    fill_group(L.g[0], *g, st, tmp, 0);
+    int tx, ty;
+    const int shape = shape_of(g->dtype, &tx, &ty);
+    tiles_for(L, tx, ty);
+    L.dt = dt_t; L.t = dt_t + g->B; L.pm = pm; L.active = nullptr;
+    L.part = (double*)((char*)ws + align_up(3 * (size_t)g->B * g->ny * g->nx * sizeof(R)));
+    int rc = launch_step(L, shape, s);
+    if (rc) return rc;
+    FinalizeArgs F{};
+    F.part = L.part; F.dt = dt_t; F.pm = pm; F.vols = vols; F.rate = rate; F.Btot = g->B; F.ngroups = 1;
+    fill_finalize(F, 0, *g, 0, 0, L.g[0].tiles_x * L.g[0].tiles_y);
+    return launch_finalize(F, s);
+}
+
+template <typename R>
+static int op_reduce(const his_grid2d* g, const void* h, const void* hu, const void* hv, const void* sfill,
+                     double* rate, double* volume, void* ws, hipStream_t s) {
+    Launch2D<R> L{};
+    L.ngroups = 1;
+    L.Btot = g->B;
+    his_state2d st{const_cast<void*>(h), const_cast<void*>(hu), const_cast<void*>(hv), const_cast<void*>(sfill),
+                   nullptr};
+    fill_group(L.g[0], *g, st, (R*)nullptr, 0);
+    int tx, ty;
+    const int shape = shape_of(g->dtype, &tx, &ty);
+    tiles_for(L, tx, ty);
+    L.part = (double*)((char*)ws + align_up(3 * (size_t)g->B * g->ny * g->nx * sizeof(R)));
+    int rc = launch_state_reduce(L, shape, s);
+    if (rc) return rc;
+    FinalizeArgs F{};
+    F.part = L.part; F.rate = rate; F.volume = volume; F.Btot = g->B; F.ngroups = 1; F.stage_only = 1;
+    fill_finalize(F, 0, *g, 0, 0, L.g[0].tiles_x * L.g[0].tiles_y);
+    return launch_finalize(F, s);
+}
+
+}  // namespace his
+
+using namespace his;
+
+// ------------------------------------------------------------------ C ABI: 2-D
+
+extern "C" size_t his_swe2d_workspace_bytes(const his_grid2d* g) {
+    if (check_grid(g)) return 0;
+    return ws2d_bytes(*g);
+}
+
+extern "C" int his_swe2d_step(const his_grid2d* g, his_state2d st, const double* dt_t, const double* pm,
+                              double* vols, double* rate, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!st.h || !st.hu || !st.hv || !st.s_filled || !st.infil_cum || !dt_t || !pm || !vols) {
+        set_error("his_swe2d_step: a state, control or output pointer is NULL");
+        return HIS_E_ARG;
+    }
+    if (!ws || ws_bytes < ws2d_bytes(*g)) {
+        set_error("his_swe2d_step: workspace of %zu bytes, %zu needed", ws_bytes, ws2d_bytes(*g));
+        return HIS_E_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    return g->dtype == HIS_F64 ? op_step<double>(g, st, dt_t, pm, vols, rate, ws, s)
+                               : op_step<float>(g, st, dt_t, pm, vols, rate, ws, s);
+}
+
+extern "C" int his_swe2d_max_rate(const his_grid2d* g, const void* h, const void* hu, const void* hv, double* rate,
+                                  void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!h || !hu || !hv || !rate) { set_error("his_swe2d_max_rate: NULL pointer"); return HIS_E_ARG; }
+    if (!ws || ws_bytes < ws2d_bytes(*g)) { set_error("his_swe2d_max_rate: workspace too small"); return HIS_E_WORKSPACE; }
+    // the reduction also sums s_filled; h stands in for it here and that sum is discarded
+    hipStream_t s = (hipStream_t)stream;
+    return g->dtype == HIS_F64 ? op_reduce<double>(g, h, hu, hv, h, rate, nullptr, ws, s)
+                               : op_reduce<float>(g, h, hu, hv, h, rate, nullptr, ws, s);
+}
+
+extern "C" int his_swe2d_volumes(const his_grid2d* g, const void* h, const void* s_filled, double* out, void* ws,
+                                 size_t ws_bytes, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!h || !s_filled || !out) { set_error("his_swe2d_volumes: NULL pointer"); return HIS_E_ARG; }
+    if (!ws || ws_bytes < ws2d_bytes(*g)) { set_error("his_swe2d_volumes: workspace too small"); return HIS_E_WORKSPACE; }
+    // the reduction also forms the CFL rate; velocities are taken as zero (hu = hv = h is not used)
+    hipStream_t s = (hipStream_t)stream;
+    return g->dtype == HIS_F64 ? op_reduce<double>(g, h, h, h, s_filled, nullptr, out, ws, s)
+                               : op_reduce<float>(g, h, h, h, s_filled, nullptr, out, ws, s);
+}
+
+extern "C" int his_swe2d_velocities(const his_grid2d* g, const void* h, const void* hu, const void* hv, void* u,
+                                    void* v, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!h || !hu || !hv || !u || !v) { set_error("his_swe2d_velocities: NULL pointer"); return HIS_E_ARG; }
+    his_state2d st{const_cast<void*>(h), const_cast<void*>(hu), const_cast<void*>(hv), nullptr, nullptr};
+    hipStream_t s = (hipStream_t)stream;
+    if (g->dtype == HIS_F64) {
+        Launch2D<double> L{};
+        L.ngroups = 1;
+        fill_group(L.g[0], *g, st, (double*)nullptr, 0);
+        return launch_velocities(L, (double*)u, (double*)v, s);
+    }
+    Launch2D<float> L{};
+    L.ngroups = 1;
+    fill_group(L.g[0], *g, st, (float*)nullptr, 0);
+    return launch_velocities(L, (float*)u, (float*)v, s);
+}
+
+extern "C" int his_plausibility(const his_grid2d* g, const void* h, const his_net1d* net, const double* A,
+                                double* out, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!h || !out || (net && !A)) { set_error("his_plausibility: NULL pointer"); return HIS_E_ARG; }
+    return launch_plaus(g->dtype, h, (int64_t)g->ny * g->nx, g->B, net, A, out, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ C ABI: 1-D and exchange
+
+static int check_net(const his_net1d* n) {
+    if (!n || n->B < 1 || n->nc < 1 || n->nf < 0 || n->ngh < 0) { set_error("bad network descriptor"); return HIS_E_ARG; }
+    if (!n->z || !n->b || !n->yb || !n->ts || !n->m || !n->nman || !n->dx || !n->kind || !n->lf_ptr || !n->rf_ptr) {
+        set_error("network descriptor has a NULL array");
+        return HIS_E_ARG;
+    }
+    if (n->nf > 0 && (!n->fl || !n->fr || !n->fb || !n->fyb || !n->fts || !n->fm || !n->lf_face || !n->rf_face)) {
+        set_error("network descriptor has a NULL face array");
+        return HIS_E_ARG;
+    }
+    return 0;
+}
+
+extern "C" size_t his_swe1d_workspace_bytes(const his_net1d* n) {
+    if (check_net(n)) return 0;
+    return swe1d_ws_doubles(*n) * sizeof(double);
+}
+
+extern "C" int his_swe1d_step(const his_net1d* n, double* A, double* Q, const double* dt, const double* pm,
+                              double* vols, void* ws, size_t ws_bytes, void* stream) {
+    int rc = check_net(n);
+    if (rc) return rc;
+    if (!A || !Q || !dt || !pm || !vols) { set_error("his_swe1d_step: NULL pointer"); return HIS_E_ARG; }
+    if (!ws || ws_bytes < swe1d_ws_doubles(*n) * sizeof(double)) {
+        set_error("his_swe1d_step: workspace too small");
+        return HIS_E_WORKSPACE;
+    }
+    Net1DDrive d{};
+    d.dt = dt; d.pm = pm; d.vols = vols; d.Btot = n->B; d.member0 = 0;
+    return launch_swe1d(*n, A, Q, d, (double*)ws, n->B, (hipStream_t)stream);
+}
+
+extern "C" int his_swe1d_max_rate(const his_net1d* n, const double* A, const double* Q, double* rate,
+                                  int32_t* any_wet, void* stream) {
+    int rc = check_net(n);
+    if (rc) return rc;
+    if (!A || !Q || !rate || !any_wet) { set_error("his_swe1d_max_rate: NULL pointer"); return HIS_E_ARG; }
+    return launch_swe1d_rate(*n, A, Q, rate, any_wet, 0, (hipStream_t)stream);
+}
+
+static int check_links(const his_links* l, const his_net1d* n) {
+    if (!l || l->nl < 0 || l->nu < 0) { set_error("bad links descriptor"); return HIS_E_ARG; }
+    if (l->nl > 0 && (!l->cell1d || !l->inv || !l->uniq || !l->zbank || !l->length || !l->z2 || !l->u_ptr ||
+                      !l->u_link || !l->c_ptr || !l->c_link)) {
+        set_error("links descriptor has a NULL array");
+        return HIS_E_ARG;
+    }
+    return check_net(n);
+}
+
+extern "C" size_t his_exchange_workspace_bytes(const his_links* l, const his_net1d* n) {
+    if (check_links(l, n)) return 0;
+    return exchange_ws_doubles(*l, *n) * sizeof(double);
+}
+
+extern "C" int his_exchange(const his_links* l, const his_net1d* n, int32_t dtype2d, int64_t plane, void* h,
+                            void* hu, void* hv, double* A, const double* dt, double* V, double* ex, void* ws,
+                            size_t ws_bytes, void* stream) {
+    int rc = check_links(l, n);
+    if (rc) return rc;
+    if (!h || !hu || !hv || !A || !dt) { set_error("his_exchange: NULL pointer"); return HIS_E_ARG; }
+    if (dtype2d != HIS_F64 && dtype2d != HIS_F32) { set_error("his_exchange: bad dtype"); return HIS_E_ARG; }
+    if (!ws || ws_bytes < exchange_ws_doubles(*l, *n) * sizeof(double)) {
+        set_error("his_exchange: workspace too small");
+        return HIS_E_WORKSPACE;
+    }
+    ExchangeArgs a{};
+    a.L = *l; a.N = *n; a.dtype2d = dtype2d; a.plane = plane; a.h = h; a.hu = hu; a.hv = hv; a.A = A; a.dt = dt;
+    a.V = V; a.acc = ex; a.acc_slot_in = 0; a.acc_stride = n->B; a.ws = (double*)ws; a.member0 = 0;
+    return launch_exchange(a, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ engine
+
+namespace his {
+
+struct DtGroup { int m0, m1, gi, pad; };
+
+struct DtArgs {
+    const DtGroup* groups;
+    int ngroups_dt;
+    double cfl, dt_min, dt_max;
+    // per grid group
+    int bc[HIS_MAX_GROUPS][4];
+    his_series rain[HIS_MAX_GROUPS], edge[HIS_MAX_GROUPS][4];
+    int member0[HIS_MAX_GROUPS];
+    int has_net[HIS_MAX_GROUPS];
+    double cfl_1d[HIS_MAX_GROUPS], dt_max_1d[HIS_MAX_GROUPS];
+    his_control ctl;
+    int32_t* active;
+    double* pm;                // (9, Btot)
+    const double* rate2d;      // (Btot)
+    const double* rate1d;      // (Btot)
+    const int32_t* wet1d;      // (Btot)
+    int Btot;
+};
+
+// One block per shared-dt group: choose dt and n_sub, mark members active, evaluate the forcing.
+__global__ void k_dt(const DtArgs a) {
+    const DtGroup G = a.groups[blockIdx.x];
+    const int gi = G.gi;
+    __shared__ double sh_dt;
+    __shared__ int sh_act;
+    if (threadIdx.x == 0) {
+        const int m0 = G.m0;
+        const double t = a.ctl.t[m0];
+        const double lim = a.ctl.t_limit[m0] < a.ctl.t_end[m0] ? a.ctl.t_limit[m0] : a.ctl.t_end[m0];
+        const int act = t < lim - 1e-9;
+        double dt = 0.0;
+        int nsub = 0;
+        if (act) {
+            double r2 = 0.0, r1 = 0.0;
+            int w1 = 0;
+            for (int m = G.m0; m < G.m1; ++m) {
+                r2 = a.rate2d[m] > r2 ? a.rate2d[m] : r2;
+                if (a.has_net[gi]) {
+                    r1 = a.rate1d[m] > r1 ? a.rate1d[m] : r1;
+                    w1 |= a.wet1d[m];
+                }
+            }
+            double d2 = r2 > 0.0 ? a.cfl / r2 : a.dt_max;
+            if (d2 < a.dt_min || d2 > a.dt_max) {
+                if (d2 < a.dt_min)
+                    for (int m = G.m0; m < G.m1; ++m) a.ctl.clamped[m] += 1;
+                d2 = d2 < a.dt_min ? a.dt_min : d2;
+                d2 = d2 > a.dt_max ? a.dt_max : d2;
+            }
+            dt = d2 < lim - t ? d2 : lim - t;
+            if (a.has_net[gi]) {
+                double d1 = a.dt_max_1d[gi];
+                if (w1) {
+                    d1 = a.cfl_1d[gi] / r1;
+                    d1 = d1 < a.dt_max_1d[gi] ? d1 : a.dt_max_1d[gi];
+                }
+                double q = ceil(dt / d1 - 1e-9);
+                nsub = q < 1.0 ? 1 : (int)q;
+            }
+        }
+        for (int m = G.m0; m < G.m1; ++m) {
+            a.active[m] = act;
+            if (act) {
+                a.ctl.dt[m] = dt;
+                a.ctl.n_sub[m] = nsub;
+                a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
+                a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
+                a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
+            }
+        }
+        sh_dt = dt;
+        sh_act = act;
+    }
+    __syncthreads();
+    if (!sh_act) return;
+    const double dt = sh_dt;
+    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
+        const int b = m - a.member0[gi];
+        const double t = a.ctl.t[m];
+        a.pm[m] = series_eval(a.rain[gi], b, t);
+        for (int e = 0; e < 4; ++e) {
+            const bool valued = a.bc[gi][e] == HIS_BC_STAGE || a.bc[gi][e] == HIS_BC_DISCHARGE;
+            a.pm[(1 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t) : 0.0;
+            a.pm[(5 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t + dt) : 0.0;
+        }
+    }
+}
+
+__global__ void k_advance(his_control ctl, const int32_t* active, int Btot) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < Btot; m += gridDim.x * blockDim.x) {
+        if (!active[m]) continue;
+        ctl.t[m] = ctl.t[m] + ctl.dt[m];
+        ctl.steps[m] += 1;
+    }
+}
+
+}  // namespace his
+
+struct his_engine {
+    his_engine_desc d;
+    int dtype, shape, Btot;
+    Launch2D<float> L32;
+    Launch2D<double> L64;
+    FinalizeArgs F;
+    DtArgs D;
+    int n_dtg;
+    his_net1d net[HIS_MAX_GROUPS];
+    his_links links[HIS_MAX_GROUPS];
+    int has_net[HIS_MAX_GROUPS];
+    double* ws1d[HIS_MAX_GROUPS];
+    double* wsx[HIS_MAX_GROUPS];
+    int32_t* active;
+    double *pm, *rate2d, *rate1d, *volume;
+    int32_t* wet1d;
+    hipStream_t cap, side;
+    hipEvent_t ev_fork, ev_join;
+    hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
+    int kmulti;
+    int graphs_ready;
+};
+
+static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
+    int rc;
+    hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(64), 0, s, e->D);
+    HIS_HIP(hipGetLastError());
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        if (!e->has_net[gi]) continue;
+        const his_group& G = e->d.group[gi];
+        ExchangeArgs a{};
+        a.L = e->links[gi]; a.N = e->net[gi]; a.dtype2d = e->dtype; a.plane = (int64_t)G.grid.ny * G.grid.nx;
+        a.h = G.state.h; a.hu = G.state.hu; a.hv = G.state.hv; a.A = G.A; a.dt = e->d.ctl.dt; a.V = nullptr;
+        a.acc = e->d.ctl.acc; a.acc_slot_in = 8; a.acc_stride = e->Btot; a.active = e->active; a.ws = e->wsx[gi];
+        a.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        if ((rc = launch_exchange(a, s))) return rc;
+    }
+    // 1-D branch beside the 2-D stages
+    bool any_net = false;
+    for (int gi = 0; gi < e->d.ngroups; ++gi) any_net |= e->has_net[gi] != 0;
+    hipStream_t s1 = any_net && side ? side : s;
+    if (any_net && side) {
+        HIS_HIP(hipEventRecord(e->ev_fork, s));
+        HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
+    }
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        if (!e->has_net[gi]) continue;
+        const his_group& G = e->d.group[gi];
+        Net1DDrive d{};
+        d.dt = e->d.ctl.dt; d.engine = 1; d.n_sub = e->d.ctl.n_sub; d.t = e->d.ctl.t;
+        d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
+        d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
+        d.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        if ((rc = launch_swe1d(e->net[gi], G.A, G.Q, d, e->ws1d[gi], G.grid.B, s1))) return rc;
+    }
+    rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
+    if (rc) return rc;
+    if ((rc = launch_finalize(e->F, s))) return rc;
+    if (any_net && side) {
+        HIS_HIP(hipEventRecord(e->ev_join, side));
+        HIS_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
+    }
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(256), 0, s, e->d.ctl, e->active, e->Btot);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
+    if (!d || d->ngroups < 1 || d->ngroups > HIS_MAX_GROUPS) return 0;
+    size_t tot = 0;
+    int Btot = 0;
+    int64_t tiles = 0;
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        const his_group& G = d->group[gi];
+        if (check_grid(&G.grid)) return 0;
+        tot += align_up(3 * (size_t)G.grid.B * G.grid.ny * G.grid.nx * esize(G.grid.dtype));
+        tiles += max_tiles(G.grid);
+        Btot += G.grid.B;
+        if (G.net) {
+            tot += align_up(swe1d_ws_doubles(*G.net) * 8);
+            if (G.links) tot += align_up(exchange_ws_doubles(*G.links, *G.net) * 8);
+        }
+    }
+    tot += align_up((size_t)tiles * NP2 * 8);
+    tot += align_up((size_t)Btot * 4) + align_up((size_t)Btot * 9 * 8) + 3 * align_up((size_t)Btot * 8) +
+           align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * sizeof(DtGroup));
+    return tot;
+}
+
+template <typename R>
+static void engine_fill_launch(his_engine* e, Launch2D<R>& L, char*& p) {
+    L.ngroups = e->d.ngroups;
+    int m0 = 0;
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        const his_group& G = e->d.group[gi];
+        R* tmp = (R*)p;
+        p += align_up(3 * (size_t)G.grid.B * G.grid.ny * G.grid.nx * sizeof(R));
+        fill_group(L.g[gi], G.grid, G.state, tmp, m0);
+        m0 += G.grid.B;
+    }
+    L.Btot = m0;
+    int tx, ty;
+    e->shape = shape_of(e->dtype, &tx, &ty);
+    tiles_for(L, tx, ty);
+}
+
+extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, void* stream) {
+    if (!d || !out) { set_error("his_engine_create: NULL argument"); return HIS_E_ARG; }
+    if (d->ngroups < 1 || d->ngroups > HIS_MAX_GROUPS) { set_error("1 to %d groups", HIS_MAX_GROUPS); return HIS_E_ARG; }
+    const size_t need = his_engine_workspace_bytes(d);
+    if (!need) return HIS_E_ARG;
+    if (!d->ws || d->ws_bytes < need) {
+        set_error("his_engine_create: workspace of %zu bytes, %zu needed", d->ws_bytes, need);
+        return HIS_E_WORKSPACE;
+    }
+    const int dtype = d->group[0].grid.dtype;
+    const int order = d->group[0].grid.order;
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        const his_group& G = d->group[gi];
+        if (G.grid.dtype != dtype || G.grid.order != order) {
+            set_error("every group of an engine must share dtype and spatial order");
+            return HIS_E_ARG;
+        }
+        if ((G.net == nullptr) != (G.links == nullptr)) { set_error("net and links go together"); return HIS_E_ARG; }
+        if (G.net && (G.net->B != G.grid.B || !G.A || !G.Q)) {
+            set_error("group %d: network batch or A/Q do not match the grid", gi);
+            return HIS_E_ARG;
+        }
+        if (G.dt_group_size < 1 || G.grid.B % G.dt_group_size) {
+            set_error("group %d: dt_group_size %d must divide B=%d", gi, G.dt_group_size, G.grid.B);
+            return HIS_E_ARG;
+        }
+        if (!G.state.h || !G.state.hu || !G.state.hv || !G.state.s_filled || !G.state.infil_cum) {
+            set_error("group %d: state pointer NULL", gi);
+            return HIS_E_ARG;
+        }
+    }
+    const his_control& c = d->ctl;
+    if (!c.t || !c.t_limit || !c.t_end || !c.dt || !c.n_sub || !c.steps || !c.clamped || !c.acc || !c.dt_min_seen ||
+        !c.dt_max_seen || !c.n_sub_max) {
+        set_error("his_engine_create: control block has a NULL array");
+        return HIS_E_ARG;
+    }
+    his_engine* e = new his_engine();
+    e->d = *d;
+    e->dtype = dtype;
+    char* p = (char*)d->ws;
+    if (dtype == HIS_F64) engine_fill_launch(e, e->L64, p);
+    else engine_fill_launch(e, e->L32, p);
+    const int Btot = e->Btot = dtype == HIS_F64 ? e->L64.Btot : e->L32.Btot;
+    const int ntiles = dtype == HIS_F64 ? e->L64.ntiles : e->L32.ntiles;
+    double* part = (double*)p;
+    int64_t tiles_cap = 0;
+    for (int gi = 0; gi < d->ngroups; ++gi) tiles_cap += max_tiles(d->group[gi].grid);
+    p += align_up((size_t)tiles_cap * NP2 * 8);
+    (void)ntiles;
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        const his_group& G = d->group[gi];
+        e->has_net[gi] = G.net != nullptr;
+        if (G.net) {
+            e->net[gi] = *G.net;
+            e->links[gi] = *G.links;
+            e->ws1d[gi] = (double*)p;
+            p += align_up(swe1d_ws_doubles(*G.net) * 8);
+            e->wsx[gi] = (double*)p;
+            p += align_up(exchange_ws_doubles(*G.links, *G.net) * 8);
+        }
+    }
+    e->active = (int32_t*)p; p += align_up((size_t)Btot * 4);
+    e->pm = (double*)p; p += align_up((size_t)Btot * 9 * 8);
+    e->rate2d = (double*)p; p += align_up((size_t)Btot * 8);
+    e->rate1d = (double*)p; p += align_up((size_t)Btot * 8);
+    p += align_up((size_t)Btot * 8);
+    e->volume = (double*)p; p += align_up((size_t)Btot * 2 * 8);
+    e->wet1d = (int32_t*)p; p += align_up((size_t)Btot * 4);
+    DtGroup* dtg_dev = (DtGroup*)p;
+    // launches
+    if (dtype == HIS_F64) {
+        e->L64.dt = c.dt; e->L64.t = c.t; e->L64.pm = e->pm; e->L64.active = e->active; e->L64.part = part;
+    } else {
+        e->L32.dt = c.dt; e->L32.t = c.t; e->L32.pm = e->pm; e->L32.active = e->active; e->L32.part = part;
+    }
+    FinalizeArgs& F = e->F;
+    F = FinalizeArgs{};
+    F.part = part; F.dt = c.dt; F.pm = e->pm; F.active = e->active; F.acc = c.acc; F.rate = e->rate2d;
+    F.volume = e->volume; F.Btot = Btot; F.ngroups = d->ngroups;
+    std::vector<DtGroup> dtg;
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        const his_group& G = d->group[gi];
+        const int m0 = dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        const int t0 = dtype == HIS_F64 ? e->L64.g[gi].tile0 : e->L32.g[gi].tile0;
+        const int per = dtype == HIS_F64 ? e->L64.g[gi].tiles_x * e->L64.g[gi].tiles_y
+                                         : e->L32.g[gi].tiles_x * e->L32.g[gi].tiles_y;
+        fill_finalize(F, gi, G.grid, m0, t0, per);
+        for (int k = 0; k < G.grid.B; k += G.dt_group_size) dtg.push_back(DtGroup{m0 + k, m0 + k + G.dt_group_size, gi, 0});
+    }
+    e->n_dtg = (int)dtg.size();
+    DtArgs& D = e->D;
+    D = DtArgs{};
+    D.groups = dtg_dev; D.ngroups_dt = e->n_dtg; D.cfl = d->cfl; D.dt_min = d->dt_min; D.dt_max = d->dt_max;
+    D.ctl = c; D.active = e->active; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
+    D.Btot = Btot;
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        const his_group& G = d->group[gi];
+        for (int k = 0; k < 4; ++k) { D.bc[gi][k] = G.grid.bc[k]; D.edge[gi][k] = G.edge[k]; }
+        D.rain[gi] = G.rain;
+        D.member0[gi] = dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        D.has_net[gi] = G.net != nullptr;
+        D.cfl_1d[gi] = G.net ? G.net->cfl_1d : 0.45;
+        D.dt_max_1d[gi] = G.net ? G.net->dt_max : d->dt_max;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t he = hipMemcpyAsync(dtg_dev, dtg.data(), dtg.size() * sizeof(DtGroup), hipMemcpyHostToDevice, s);
+    if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(dt groups)"); }
+    he = hipStreamSynchronize(s);
+    if (he != hipSuccess) { delete e; return hip_fail(he, "hipStreamSynchronize"); }
+    if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
+        delete e;
+        set_error("his_engine_create: stream/event creation failed");
+        return HIS_E_HIP;
+    }
+    *out = e;
+    int rc = his_engine_refresh(e, stream);
+    if (rc) return rc;
+    return 0;
+}
+
+extern "C" int his_engine_refresh(his_engine* e, void* stream) {
+    if (!e) { set_error("NULL engine"); return HIS_E_ARG; }
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (e->dtype == HIS_F64) rc = launch_state_reduce(e->L64, e->shape, s);
+    else rc = launch_state_reduce(e->L32, e->shape, s);
+    if (rc) return rc;
+    FinalizeArgs F = e->F;
+    F.stage_only = 1;
+    F.active = nullptr;
+    F.acc = nullptr;
+    if ((rc = launch_finalize(F, s))) return rc;
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        if (!e->has_net[gi]) continue;
+        const his_group& G = e->d.group[gi];
+        const int m0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        if ((rc = launch_swe1d_rate(e->net[gi], G.A, G.Q, e->rate1d, e->wet1d, m0, s))) return rc;
+    }
+    return 0;
+}
+
+static int engine_build_graphs(his_engine* e) {
+    e->kmulti = 8;
+    for (int k = 0; k < 2; ++k) {
+        const int n = k == 0 ? 1 : e->kmulti;
+        hipGraph_t g;
+        HIS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));
+        int rc = 0;
+        for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side);
+        hipError_t he = hipStreamEndCapture(e->cap, &g);
+        if (rc) return rc;
+        if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture");
+        he = hipGraphInstantiate(&e->exec[k], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (he != hipSuccess) return hip_fail(he, "hipGraphInstantiate");
+    }
+    e->graphs_ready = 1;
+    return 0;
+}
+
+extern "C" int his_engine_step(his_engine* e, int32_t n_steps, void* stream) {
+    if (!e || n_steps < 0) { set_error("his_engine_step: bad argument"); return HIS_E_ARG; }
+    hipStream_t s = (hipStream_t)stream;
+    if (!e->d.use_graph) {
+        for (int i = 0; i < n_steps; ++i) {
+            int rc = engine_enqueue_step(e, s, e->side);
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    if (!e->graphs_ready) {
+        int rc = engine_build_graphs(e);
+        if (rc) return rc;
+    }
+    int left = n_steps;
+    while (left >= e->kmulti) {
+        HIS_HIP(hipGraphLaunch(e->exec[1], s));
+        left -= e->kmulti;
+    }
+    while (left-- > 0) HIS_HIP(hipGraphLaunch(e->exec[0], s));
+    return 0;
+}
+
+extern "C" int his_engine_fold_max(his_engine* e, int32_t group, void* hmax, void* stream) {
+    if (!e || group < 0 || group >= e->d.ngroups || !hmax) { set_error("his_engine_fold_max: bad argument"); return HIS_E_ARG; }
+    hipStream_t s = (hipStream_t)stream;
+    if (e->dtype == HIS_F64) return launch_fold_max(e->L64, group, (double*)hmax, s);
+    return launch_fold_max(e->L32, group, (float*)hmax, s);
+}
+
+extern "C" int his_engine_destroy(his_engine* e) {
+    if (!e) return 0;
+    if (e->graphs_ready) {
+        (void)hipGraphExecDestroy(e->exec[0]);
+        (void)hipGraphExecDestroy(e->exec[1]);
+    }
+    (void)hipStreamDestroy(e->cap);
+    (void)hipStreamDestroy(e->side);
+    (void)hipEventDestroy(e->ev_fork);
+    (void)hipEventDestroy(e->ev_join);
+    delete e;
+    return 0;
+}
+
+extern "C" int his_set_tile_shape(int32_t dtype, int32_t shape) {
+    if ((dtype != HIS_F64 && dtype != HIS_F32) || shape < 0 || shape >= kNumShapes) {
+        set_error("his_set_tile_shape: bad dtype or shape");
+        return HIS_E_ARG;
+    }
+    g_shape[dtype == HIS_F64 ? 0 : 1] = shape;
+    return 0;
+}
+
+extern "C" int his_tile_shape(int32_t dtype, int32_t shape, int32_t* tx, int32_t* ty) {
+    if ((dtype != HIS_F64 && dtype != HIS_F32) || shape < 0 || shape >= kNumShapes || !tx || !ty) return HIS_E_ARG;
+    const TileShape& t = dtype == HIS_F64 ? kShapesF64[shape] : kShapesF32[shape];
+    *tx = t.tx;
+    *ty = t.ty;
+    return 0;
+}
+
+extern "C" const char* his_last_error(void) { return g_err; }
+extern "C" const char* his_version(void) { return "his_amd 0.1 gfx950"; }

@@ -1,0 +1,460 @@
+// 1-D St. Venant network step and Villemonte weir exchange (float64, gfx950).
+//
+// One workgroup per member keeps the whole network in flight: the reference's
+// gathers become per-face reads, and every index_add becomes a per-cell loop over
+// that cell's faces (or links) in increasing index order -- the order torch's CPU
+// index_add accumulates in -- so results are deterministic and follow the reference
+// bit for bit wherever the libm functions agree.
+//
+// SWE1D._step_core  hydrointel/solver/swe1d.py:320-368 (+ _fill_ghosts 265-294,
+//                   Sections 50-84, hll_1d 87-107, max_dt 253-262)
+// exchange          hydrointel/solver/coupling.py:73-124
+#include "net1d.hpp"
+
+namespace his {
+
+struct Sec { double b, m, yb, ts; };
+
+__device__ __forceinline__ double bank_area(const Sec& s) { return s.yb * (s.b + s.m * s.yb); }
+__device__ __forceinline__ double sec_area(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    double at = yt * (s.b + s.m * yt);
+    return at + s.ts * clamp0(y - s.yb);
+}
+__device__ __forceinline__ double sec_top(const Sec& s, double y) { return y <= s.yb ? s.b + 2.0 * s.m * y : s.ts; }
+__device__ __forceinline__ double sec_i1(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    double above = clamp0(y - s.yb);
+    double i_t = s.b * yt * yt / 2.0 + s.m * (yt * yt * yt) / 3.0;
+    double at = yt * (s.b + s.m * yt);
+    return i_t + at * above + s.ts * above * above / 2.0;
+}
+__device__ __forceinline__ double sec_perim(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    return s.b + 2.0 * yt * sqrt(1.0 + s.m * s.m);
+}
+__device__ __forceinline__ double sec_depth(const Sec& s, double A) {
+    A = clamp0(A);
+    double ab = bank_area(s);
+    double y_t = 2.0 * A / (s.b + sqrt(s.b * s.b + 4.0 * s.m * A));
+    return A <= ab ? y_t : s.yb + (A - ab) / s.ts;
+}
+__device__ __forceinline__ Sec cell_sec(const his_net1d& N, int b, int c) {
+    const int64_t o = (int64_t)b * N.bs + c;
+    return Sec{N.b[o], N.m[c], N.yb[o], N.ts[o]};
+}
+__device__ __forceinline__ Sec face_sec(const his_net1d& N, int b, int f) {
+    const int64_t o = (int64_t)b * N.fbs + f;
+    return Sec{N.fb[o], N.fm[f], N.fyb[o], N.fts[o]};
+}
+__device__ __forceinline__ double vel1(const Sec& s, double A, double Q, double h_dry) {
+    double y = sec_depth(s, A);
+    return y >= h_dry ? Q / (A < 1e-12 ? 1e-12 : A) : 0.0;
+}
+
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum_bcast(double (&v)[NV], double (*red)[8]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < NV; ++k) red[w][k] = v[k];
+    __syncthreads();
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int w2 = 0; w2 < NT / 64; ++w2) s += red[w2][k];
+        v[k] = s;
+    }
+}
+
+// per-member epilogue: rate of the current state (swe1d.py:253-262)
+template <int NT>
+__device__ void rate1d(const his_net1d& N, int b, const double* A, const double* Q, double* rate, int32_t* wet,
+                       int m, double (*red)[8]) {
+    double mx = 0.0, anyw = 0.0;
+    for (int c = threadIdx.x; c < N.nc; c += NT) {
+        if (N.kind[c] >= 2) continue;
+        Sec s = cell_sec(N, b, c);
+        double a = A[c], q = Q[c];
+        double y = sec_depth(s, a);
+        if (!(y > N.h_dry)) continue;
+        anyw = 1.0;
+        double cc = sqrt(N.g * a / sec_top(s, y));
+        double u = vel1(s, a, q, N.h_dry);
+        double r = (fabs(u) + cc) / N.dx[c];
+        mx = r > mx ? r : mx;
+    }
+    mx = wave_max(mx);
+    anyw = wave_max(anyw);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { red[w][0] = mx; red[w][1] = anyw; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, bw = 0.0;
+        for (int w2 = 0; w2 < NT / 64; ++w2) {
+            a = red[w2][0] > a ? red[w2][0] : a;
+            bw = red[w2][1] > bw ? red[w2][1] : bw;
+        }
+        if (rate) rate[m] = a;
+        if (wet) wet[m] = bw > 0.0 ? 1 : 0;
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restrict__ Ag, double* __restrict__ Qg,
+                                               const Net1DDrive d, double* __restrict__ wsg) {
+    __shared__ double red[NT / 64][8];
+    const int b = blockIdx.x;
+    const int m = d.member0 + b;
+    if (d.active && !d.active[m]) return;
+    const int nc = N.nc, nf = N.nf;
+    double* A = Ag + (int64_t)b * nc;
+    double* Q = Qg + (int64_t)b * nc;
+    double* ws = wsg + (int64_t)b * (3 * nc + 4 * nf);
+    double *y = ws, *u = ws + nc, *th = ws + 2 * nc;
+    double *F1 = ws + 3 * nc, *F2 = F1 + nf, *pL = F2 + nf, *pR = pL + nf;
+    const double g = N.g, h_dry = N.h_dry;
+    int nsub = 1;
+    double dt_full = d.dt[m], t0 = 0.0;
+    if (d.engine) {
+        nsub = d.n_sub[m];
+        t0 = d.t[m];
+    }
+    const double sub = d.engine ? dt_full / nsub : dt_full;
+    for (int k = 0; k < nsub; ++k) {
+        double q_in, eta;
+        if (d.engine) {
+            const double tk = t0 + k * sub;
+            q_in = series_eval(d.inflow, b, tk);
+            eta = series_eval(d.stage, b, tk);
+        } else {
+            q_in = d.pm[m];
+            eta = d.pm[d.Btot + m];
+        }
+        const double dt = sub;
+        // ghosts from their interior neighbours (interior cells are not written here)
+        for (int kg = threadIdx.x; kg < N.ngh; kg += NT) {
+            const int gi = N.g_idx[kg], src = N.g_src[kg];
+            const double sg = N.g_sign[kg];
+            const Sec s = cell_sec(N, b, gi);
+            const double As = A[src], Qs = Q[src];
+            const double ys = sec_depth(s, As);
+            const double zg = N.z[(int64_t)b * N.bs + gi];
+            const double un = sg * vel1(s, As, Qs, h_dry);
+            const bool wet_i = ys >= h_dry;
+            const double yst = clamp0(eta - zg);
+            double Ast = sec_area(s, yst);
+            const double ci = sqrt(g * As / sec_top(s, ys));
+            const double cg = sqrt(g * Ast / sec_top(s, yst));
+            double un_st = wet_i ? un + 2.0 * (ci - cg) : 0.0;
+            if (wet_i && (un > ci)) { un_st = un; Ast = As; }
+            if (!(sec_depth(s, Ast) >= h_dry)) un_st = 0.0;
+            const double Q_st = sg * Ast * un_st;
+            double yc = pow(q_in * q_in / (g * s.b * s.b), 1.0 / 3.0);
+            if (yc < h_dry) yc = h_dry;
+            const double A_in = wet_i ? As : sec_area(s, yc);
+            const double Q_in = -sg * q_in;
+            const int kd = N.kind[gi];
+            A[gi] = kd == 3 ? Ast : (kd == 2 ? A_in : As);
+            Q[gi] = kd == 3 ? Q_st : (kd == 2 ? Q_in : -Qs);
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < nc; c += NT) {
+            const Sec s = cell_sec(N, b, c);
+            y[c] = sec_depth(s, A[c]);
+            u[c] = vel1(s, A[c], Q[c], h_dry);
+        }
+        __syncthreads();
+        for (int f = threadIdx.x; f < nf; f += NT) {
+            const int l = N.fl[f], r = N.fr[f];
+            const Sec fs = face_sec(N, b, f);
+            const double zL = N.z[(int64_t)b * N.bs + l], zR = N.z[(int64_t)b * N.bs + r];
+            const double zf = tmax(zL, zR);
+            const double yL = clamp0(zL + y[l] - zf), yR = clamp0(zR + y[r] - zf);
+            const double AL = sec_area(fs, yL), AR = sec_area(fs, yR);
+            const double uL = u[l], uR = u[r];
+            const double pl = g * sec_i1(fs, yL), pr = g * sec_i1(fs, yR);
+            const double cL = sqrt(g * AL / sec_top(fs, yL)), cR = sqrt(g * AR / sec_top(fs, yR));
+            const double QL = AL * uL, QR = AR * uR;
+            const bool dryL = AL <= HIS_TINY, dryR = AR <= HIS_TINY;
+            double SL = tmin(uL - cL, uR - cR), SR = tmax(uL + cL, uR + cR);
+            if (dryL) { SL = uR - 2.0 * cR; SR = uR + cR; }
+            if (dryR) { SL = uL - cL; SR = uL + 2.0 * cL; }
+            const double FL2 = QL * uL + pl, FR2 = QR * uR + pr;
+            double den = SR - SL;
+            if (fabs(den) < HIS_TINY) den = 1.0;
+            double f1, f2;
+            if (SL >= 0.0) { f1 = QL; f2 = FL2; }
+            else if (SR <= 0.0) { f1 = QR; f2 = FR2; }
+            else {
+                f1 = (SR * QL - SL * QR + SL * SR * (AR - AL)) / den;
+                f2 = (SR * FL2 - SL * FR2 + SL * SR * (QR - QL)) / den;
+            }
+            if (dryL && dryR) { f1 = 0.0; f2 = 0.0; }
+            F1[f] = f1; F2[f] = f2; pL[f] = pl; pR[f] = pr;
+        }
+        __syncthreads();
+        // conservative outflow limiter (donor = upwind cell; ghosts unlimited)
+        for (int c = threadIdx.x; c < nc; c += NT) {
+            double out = 0.0;
+            for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) out += relu(F1[N.lf_face[p]]);
+            for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) out += relu(-F1[N.rf_face[p]]);
+            const double vol = A[c] * N.dx[c];
+            const double need = out * dt;
+            const double t = need > vol ? vol / (need > 0.0 ? need : 1.0) : 1.0;
+            th[c] = N.kind[c] >= 2 ? 1.0 : t;
+        }
+        __syncthreads();
+        double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // clip, in(gl), in(gr), out(gl), out(gr)
+        for (int f = threadIdx.x; f < nf; f += NT) {
+            const double f1 = F1[f];
+            const double sc = f1 > 0.0 ? th[N.fl[f]] : th[N.fr[f]];
+            const double f1s = f1 * sc;
+            F1[f] = f1s;
+            const bool gl = N.kind[N.fl[f]] >= 2, gr = N.kind[N.fr[f]] >= 2;
+            if (gl) { part[1] += relu(f1s); part[3] += relu(-f1s); }
+            if (gr) { part[2] += relu(-f1s); part[4] += relu(f1s); }
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < nc; c += NT) {
+            double dA = 0.0, dQ = 0.0;
+            for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) dA += -F1[N.lf_face[p]];
+            for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) dA += F1[N.rf_face[p]];
+            for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) {
+                const int f = N.lf_face[p];
+                dQ += -(F2[f] - pL[f]);
+            }
+            for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) {
+                const int f = N.rf_face[p];
+                dQ += F2[f] - pR[f];
+            }
+            const int kd = N.kind[c];
+            if (kd >= 2) continue;   // ghosts keep their (refilled) values
+            double A1 = A[c] + dt * dA / N.dx[c];
+            double Q1 = Q[c] + dt * dQ / N.dx[c];
+            const double neg = clamp0(-A1);
+            part[0] += neg * N.dx[c];
+            A1 = A1 + neg;
+            const Sec s = cell_sec(N, b, c);
+            const double y1 = sec_depth(s, A1);
+            const double Ac = A1 < 1e-12 ? 1e-12 : A1;
+            const double Rh = Ac / sec_perim(s, y1);
+            const double Rc = Rh < h_dry ? h_dry : Rh;
+            const double nm = N.nman[c];
+            const double denom = 1.0 + dt * g * (nm * nm) * fabs(Q1) / (Ac * pow(Rc, 4.0 / 3.0));
+            Q1 = Q1 / denom;
+            if (!((y1 >= h_dry) && kd != 1)) Q1 = 0.0;
+            A[c] = A1;
+            Q[c] = Q1;
+        }
+        block_sum_bcast<NT, 5>(part, red);
+        if (threadIdx.x == 0) {
+            const double bin = (part[1] + part[2]) * dt, bout = (part[3] + part[4]) * dt;
+            if (d.engine) {
+                d.acc[4 * d.Btot + m] += bin;
+                d.acc[5 * d.Btot + m] += bout;
+                d.acc[7 * d.Btot + m] += part[0];
+            } else if (d.vols) {
+                d.vols[m] = bin;
+                d.vols[d.Btot + m] = bout;
+                d.vols[2 * d.Btot + m] = part[0];
+            }
+        }
+        __syncthreads();
+    }
+    if (d.rate || d.wet) rate1d<NT>(N, b, A, Q, d.rate, d.wet, m, red);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const double* Ag, const double* Qg,
+                                                    double* rate, int32_t* wet, int member0) {
+    __shared__ double red[NT / 64][8];
+    const int b = blockIdx.x;
+    rate1d<NT>(N, b, Ag + (int64_t)b * N.nc, Qg + (int64_t)b * N.nc, rate, wet, member0 + b, red);
+}
+
+size_t swe1d_ws_doubles(const his_net1d& n) { return (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf); }
+
+int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_swe1d<256>, dim3(nblocks), dim3(256), 0, s, n, A, Q, d, ws);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
+                      int member0, hipStream_t s) {
+    hipLaunchKernelGGL(k_swe1d_rate<256>, dim3(n.B), dim3(256), 0, s, n, A, Q, rate, wet, member0);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ exchange
+
+size_t exchange_ws_doubles(const his_links& l, const his_net1d& n) {
+    return (size_t)n.B * (2 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc);
+}
+
+template <typename R2, int NT>
+__global__ void __launch_bounds__(NT) k_exchange(const ExchangeArgs a) {
+    __shared__ double red[NT / 64][8];
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    const int b = blockIdx.x;
+    const int m = a.member0 + b;
+    if (a.active && !a.active[m]) return;
+    const int nl = L.nl, nu = L.nu, nc = N.nc;
+    R2* h = reinterpret_cast<R2*>(a.h) + (int64_t)b * a.plane;
+    R2* hu = reinterpret_cast<R2*>(a.hu) + (int64_t)b * a.plane;
+    R2* hv = reinterpret_cast<R2*>(a.hv) + (int64_t)b * a.plane;
+    double* A = a.A + (int64_t)b * nc;
+    double* ws = a.ws + (int64_t)b * (2 * nl + 2 * nu + nc);
+    double *V = ws, *sg = ws + nl, *hu_ = ws + 2 * nl, *f2 = hu_ + nu, *f1 = f2 + nu;
+    const double dt = a.dt[m];
+    for (int k = threadIdx.x; k < nu; k += NT) hu_[k] = (double)h[L.uniq[k]];
+    __syncthreads();
+    for (int l = threadIdx.x; l < nl; l += NT) {
+        const int c1 = L.cell1d[l];
+        const Sec s = cell_sec(N, b, c1);
+        const double h2 = hu_[L.inv[l]];
+        const double eta2 = L.z2[l] + h2;
+        const double y1 = sec_depth(s, A[c1]);
+        const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
+        const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
+        const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
+        const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
+        const double q = L.weir_cw * L.length[l] * pow(d_up, 1.5) * pow(clamp0(1.0 - pow(ratio, 1.5)), 0.385);
+        sg[l] = eta2 > eta1 ? 1.0 : -1.0;
+        const double v = q * dt;
+        const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
+        const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
+        V[l] = tmin(v, L.relax * v_eq);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nu; k += NT) {
+        double tot = 0.0;
+        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
+            const int l = L.u_link[p];
+            tot += sg[l] > 0.0 ? V[l] : 0.0;
+        }
+        const double av = L.limit * hu_[k] * L.area2;
+        f2[k] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+    }
+    for (int c = threadIdx.x; c < nc; c += NT) {
+        double tot = 0.0;
+        for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
+            const int l = L.c_link[p];
+            tot += sg[l] < 0.0 ? V[l] : 0.0;
+        }
+        const Sec s = cell_sec(N, b, c);
+        const double av = L.limit * clamp0(A[c] - bank_area(s)) * N.dx[c];
+        f1[c] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+    }
+    __syncthreads();
+    double part[2] = {0.0, 0.0};
+    for (int l = threadIdx.x; l < nl; l += NT) {
+        const double w = sg[l] > 0.0 ? V[l] * f2[L.inv[l]] : V[l] * f1[L.cell1d[l]];
+        const double v = w * sg[l];
+        V[l] = v;
+        if (a.V) a.V[(int64_t)b * nl + l] = v;
+        part[0] += clamp0(v);
+        part[1] += clamp0(-v);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nu; k += NT) {
+        double dV = 0.0;
+        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) dV += V[L.u_link[p]];
+        const double h0 = hu_[k];
+        const double hn = h0 - dV / L.area2;
+        const double keep = h0 > 0.0 ? tmin(hn / (h0 > 0.0 ? h0 : 1.0), 1.0) : 0.0;
+        const int64_t q = L.uniq[k];
+        const R2 kp = (R2)keep;
+        h[q] = (R2)hn;
+        hu[q] = hu[q] * kp;
+        hv[q] = hv[q] * kp;
+    }
+    for (int c = threadIdx.x; c < nc; c += NT) {
+        const int p0 = L.c_ptr[c], p1 = L.c_ptr[c + 1];
+        if (p0 == p1) continue;
+        double add = 0.0;
+        for (int p = p0; p < p1; ++p) add += V[L.c_link[p]];
+        A[c] = A[c] + add / N.dx[c];
+    }
+    if (a.acc) {
+        block_sum_bcast<NT, 2>(part, red);
+        if (threadIdx.x == 0) {
+            a.acc[(int64_t)a.acc_slot_in * a.acc_stride + m] += part[0];
+            a.acc[(int64_t)(a.acc_slot_in + 1) * a.acc_stride + m] += part[1];
+        }
+    }
+}
+
+int launch_exchange(const ExchangeArgs& a, hipStream_t s) {
+    if (a.dtype2d == HIS_F64)
+        hipLaunchKernelGGL((k_exchange<double, 256>), dim3(a.N.B), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_exchange<float, 256>), dim3(a.N.B), dim3(256), 0, s, a);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ plausibility
+
+template <typename R2, int NT>
+__global__ void __launch_bounds__(NT) k_plaus(const R2* h, int64_t plane, const his_net1d N, const double* Ag,
+                                               double* out, int B) {
+    __shared__ double red[NT / 64][8];
+    const int b = blockIdx.x;
+    double dmax = -1e300, rise = -1e300, nonfin = 0.0;
+    for (int64_t i = threadIdx.x; i < plane; i += NT) {
+        const double v = (double)h[(int64_t)b * plane + i];
+        if (!isfinite(v)) nonfin = 1.0;
+        dmax = v > dmax ? v : dmax;
+    }
+    if (Ag) {
+        for (int c = threadIdx.x; c < N.nc; c += NT) {
+            if (N.kind[c] >= 2) continue;
+            const Sec s = cell_sec(N, b, c);
+            const double a = Ag[(int64_t)b * N.nc + c];
+            const double zb = N.z[(int64_t)b * N.bs + c];
+            const double lev = zb + sec_depth(s, a);
+            const double r = lev - N.zbank[c];
+            if (!isfinite(r)) nonfin = 1.0;
+            rise = r > rise ? r : rise;
+        }
+    }
+    dmax = wave_max(dmax);
+    rise = wave_max(rise);
+    nonfin = wave_max(nonfin);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w][0] = dmax; red[w][1] = rise; red[w][2] = nonfin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = -1e300, a1 = -1e300, a2 = 0.0;
+        for (int w2 = 0; w2 < NT / 64; ++w2) {
+            a0 = red[w2][0] > a0 ? red[w2][0] : a0;
+            a1 = red[w2][1] > a1 ? red[w2][1] : a1;
+            a2 = red[w2][2] > a2 ? red[w2][2] : a2;
+        }
+        out[b] = a2 > 0.0 ? __builtin_nan("") : a0;
+        out[B + b] = a2 > 0.0 ? __builtin_nan("") : (Ag ? a1 : -1e9);
+    }
+}
+
+int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d* N, const double* A, double* out,
+                 hipStream_t s) {
+    his_net1d dummy{};
+    const his_net1d& n = N ? *N : dummy;
+    if (dtype == HIS_F64)
+        hipLaunchKernelGGL((k_plaus<double, 256>), dim3(B), dim3(256), 0, s, (const double*)h, plane, n,
+                           N ? A : nullptr, out, B);
+    else
+        hipLaunchKernelGGL((k_plaus<float, 256>), dim3(B), dim3(256), 0, s, (const float*)h, plane, n,
+                           N ? A : nullptr, out, B);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace his

@@ -1,0 +1,52 @@
+// Internal descriptors of the 1-D network and weir-exchange kernels.
+#pragma once
+#include "common.hpp"
+
+namespace his {
+
+// Where a 1-D launch takes its per-member dt, substep count and forcing from.
+struct Net1DDrive {
+    // per-op mode: one substep of dt[m] with pm (2, Btot) = q_in, eta; vols (3, Btot) written
+    const double *dt, *pm;
+    double *vols;
+    // engine mode (engine != 0): n_sub[m] substeps of dt[m]/n_sub[m] from t[m], forcing from the
+    // series, ledger acc (10, Btot) += bnd1d_in (4), bnd1d_out (5), clip1d (7)
+    int engine;
+    const int32_t *n_sub;
+    const double *t;
+    his_series inflow, stage;
+    double *acc;
+    const int32_t *active;
+    // epilogue: CFL rate of the final state (may be NULL)
+    double *rate;
+    int32_t *wet;
+    int Btot, member0;
+};
+
+int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
+                 hipStream_t s);
+int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
+                      int member0, hipStream_t s);
+
+struct ExchangeArgs {
+    his_links L;
+    his_net1d N;
+    int dtype2d;
+    int64_t plane;
+    void *h, *hu, *hv;
+    double* A;
+    const double* dt;        // (Btot)
+    double* V;               // (B, nl) or NULL
+    double* acc;             // ledger (10, Btot): += exchanged in (8), out (9); or NULL
+    int acc_slot_in;         // 8 for the ledger, 0 for a plain (2, B) output
+    int acc_stride;          // Btot of that array
+    const int32_t* active;
+    double* ws;
+    int member0;
+};
+int launch_exchange(const ExchangeArgs& a, hipStream_t s);
+
+size_t swe1d_ws_doubles(const his_net1d& n);
+size_t exchange_ws_doubles(const his_links& l, const his_net1d& n);
+
+}  // namespace his

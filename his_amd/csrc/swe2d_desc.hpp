@@ -1,0 +1,94 @@
+// Internal launch descriptors of the fused 2-D finite-volume kernels.
+#pragma once
+#include "common.hpp"
+
+namespace his {
+
+// Per-tile float64 partial sums, one block of NP slots per stage (stage-1 block, then stage-2 block).
+enum {
+    P_CLIP = 0,                 // sum of depth added by the h >= 0 clip
+    P_WIN, P_EIN, P_SIN, P_NIN, // relu of inflow mass flux through the W, E, S, N boundary faces
+    P_WOUT, P_EOUT, P_SOUT, P_NOUT,
+    P_INF,                      // infiltrated depth
+    P_RAINW,                    // rain depth added (rain_weight runs only)
+    P_VOLH, P_VOLS,             // sum of h, sum of s_filled (new state)
+    P_RATE,                     // CFL rate maximum (new state)
+    P_USED
+};
+constexpr int NP = 16;          // slots per stage block
+constexpr int NP2 = 2 * NP;     // slots per tile
+
+template <typename R>
+struct Group2D {
+    int B, ny, nx, order, flux;
+    int bc[4];
+    int tiles_x, tiles_y, tile0, member0;
+    R dx, dy, g, h_dry, kp_eps;
+    double dxd, dyd, gd, t0, area;
+    const R *z, *n, *S, *f0, *fc, *k, *rw;
+    int64_t n_bs, S_bs, hz_bs, rw_bs;
+    R *h, *hu, *hv, *s, *ic;      // state, updated in place by the last stage
+    R *h1, *hu1, *hv1;            // intermediate stage state (or the order-1 result)
+};
+
+template <typename R>
+struct Launch2D {
+    Group2D<R> g[HIS_MAX_GROUPS];
+    int ngroups, ntiles, Btot;
+    int tx, ty;                   // tile shape used to build tiles_x / tiles_y
+    const double *dt, *t, *pm;    // (Btot), (Btot), (9, Btot)
+    const int32_t *active;        // (Btot) or NULL = every member steps
+    double *part;                 // (ntiles, NP2)
+};
+
+// tile shapes compiled for each precision; index into this table is the tuning knob
+struct TileShape { int tx, ty, nt; };
+constexpr TileShape kShapesF32[] = {{64, 8, 256}, {64, 4, 256}, {64, 16, 256}, {32, 8, 256}};
+constexpr TileShape kShapesF64[] = {{32, 8, 256}, {64, 4, 256}, {64, 8, 256}, {32, 4, 128}};
+constexpr int kNumShapes = 4;
+
+// host launchers, defined once per precision (swe2d_f32.hip, swe2d_f64.hip)
+int launch_step(const Launch2D<float>& L, int shape, hipStream_t s);
+int launch_step(const Launch2D<double>& L, int shape, hipStream_t s);
+int launch_state_reduce(const Launch2D<float>& L, int shape, hipStream_t s);   // rate + volumes into stage-2 slots
+int launch_state_reduce(const Launch2D<double>& L, int shape, hipStream_t s);
+int launch_velocities(const Launch2D<float>& L, float* u, float* v, hipStream_t s);
+int launch_velocities(const Launch2D<double>& L, double* u, double* v, hipStream_t s);
+int launch_fold_max(const Launch2D<float>& L, int group, float* hmax, hipStream_t s);
+int launch_fold_max(const Launch2D<double>& L, int group, double* hmax, hipStream_t s);
+
+// Per member: reduce the member's tile partials in fixed order.
+//   vols (5, Btot):  bnd_in, bnd_out, rain, infil, clip     (may be NULL)
+//   acc  (10, Btot): += rain, infil, bnd_in, bnd_out at slots 0, 1, 2, 3 and clip at slot 6 (may be NULL)
+//   rate (Btot), volume (2, Btot)                            (may be NULL)
+struct FinalizeArgs {
+    const double* part;
+    const double *dt, *pm;
+    const int32_t* active;
+    double *vols, *acc, *rate, *volume;
+    int Btot;
+    int ngroups;
+    int member0[HIS_MAX_GROUPS], B[HIS_MAX_GROUPS], tile0[HIS_MAX_GROUPS], per[HIS_MAX_GROUPS];
+    int order[HIS_MAX_GROUPS], weighted[HIS_MAX_GROUPS];
+    double dx[HIS_MAX_GROUPS], dy[HIS_MAX_GROUPS], area[HIS_MAX_GROUPS], cells[HIS_MAX_GROUPS];
+    int rain_f32[HIS_MAX_GROUPS];        // rain * dt formed in float32 (the working dtype) before widening
+    int stage_only;                      // 1 = only rate/volume (after a state reduction)
+};
+int launch_finalize(const FinalizeArgs& f, hipStream_t s);
+
+template <typename R>
+inline void tiles_for(Launch2D<R>& L, int tx, int ty) {
+    int t = 0;
+    L.tx = tx;
+    L.ty = ty;
+    for (int gi = 0; gi < L.ngroups; ++gi) {
+        Group2D<R>& G = L.g[gi];
+        G.tiles_x = (G.nx + tx - 1) / tx;
+        G.tiles_y = (G.ny + ty - 1) / ty;
+        G.tile0 = t;
+        t += G.B * G.tiles_x * G.tiles_y;
+    }
+    L.ntiles = t;
+}
+
+}  // namespace his

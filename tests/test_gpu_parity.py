@@ -1,0 +1,308 @@
+"""HIP kernels against the reference's golden vectors and the float64 oracle.
+
+Golden files (tests/golden/*.npz) hold the reference solver's own float64 outputs on
+seeded inputs; the oracle (oracle/) is the float64 CPU restatement pinned to them.
+Tolerance for float64: every cell within 1e-9 relative (plus 1e-13 absolute for
+values at round-off level), the bar the port is held to.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from _helpers import assert_rel
+
+pytestmark = pytest.mark.gpu
+RTOL, ATOL = 1e-9, 1e-13
+
+
+def _f64(t):
+    return t.detach().double().cpu().numpy()
+
+
+def _swe2d_mixed(d, device, dtype=torch.float64):
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import EdgeBC
+    from his_amd.solver.infiltration import Horton
+    from his_amd.solver.swe2d import SWE2D
+    B = int(d["B"])
+    stage = W.sine_tide(0.8, W.TIDE_PERIOD_S, [0.3, 2.1])
+    qin = W.triangle(0.0, 600.0, [0.4, 0.9], base=0.05)
+    bcs = {"west": EdgeBC("transmissive"), "east": EdgeBC("stage", stage), "south": EdgeBC("discharge", qin),
+           "north": EdgeBC("closed")}
+    T = lambda a: torch.as_tensor(a, dtype=dtype, device=device)
+    hz = Horton(T(d["f0"]), T(d["fc"]), T(d["k"]), float(d["t0"]))
+    return SWE2D(d["z"], d["n"], float(d["dx"]), SolverConfig(), bcs, device, dtype, storage=d["S"], horton=hz,
+                 batch=B)
+
+
+def test_swe2d_step_core_matches_reference(golden, gpu):
+    d = golden("swe2d_mixed")
+    sim = _swe2d_mixed(d, gpu)
+    st = d["states"]
+    for s in range(3):
+        rate = _f64(sim.member_rates(*(torch.as_tensor(st[s][k], device=gpu) for k in range(3))))
+        assert_rel(rate, d[f"rate_member_{s}"], RTOL, 0, f"step {s} CFL rate")
+        ins = [torch.as_tensor(st[s][k], device=gpu) for k in range(5)]
+        sc = torch.tensor([d["dt"][s], d["t"][s]], dtype=torch.float64)
+        pm = torch.as_tensor(d["pm"][s]).reshape(9, -1, 1, 1)
+        out = sim._core(*ins, sc, pm)
+        for k, name in enumerate(("h", "hu", "hv", "s_filled", "infil_cum")):
+            assert_rel(_f64(out[k]), st[s + 1][k], RTOL, ATOL, f"step {s} {name}")
+        vols = np.stack([_f64(v) for v in out[5:]])
+        assert_rel(vols, d["vols"][s], RTOL, 1e-9, f"step {s} volumes")
+        # the inputs are untouched (functional slot)
+        assert_rel(_f64(ins[0]), st[s][0], 0, 0, "input h unchanged")
+
+
+def test_swe2d_order1_hll_step_matches_reference(golden, gpu):
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import EdgeBC
+    from his_amd.solver.swe2d import SWE2D
+    d = golden("swe2d_o1hll")
+    cfg = SolverConfig(spatial_order=1, flux="hll")
+    bcs = {"west": EdgeBC("closed"), "east": EdgeBC("transmissive"), "south": EdgeBC("stage", 1.2),
+           "north": EdgeBC("discharge", -0.3)}
+    sim = SWE2D(d["z"], float(d["manning"]), float(d["dx"]), cfg, bcs, gpu, torch.float64, storage=d["S"],
+                rain_weight=d["rw"])
+    st = d["states"]
+    for s in range(2):
+        sim.h, sim.hu, sim.hv, sim.s_filled, sim.infil_cum = (torch.as_tensor(st[s][k], device=gpu) for k in range(5))
+        assert abs(sim.max_dt() - d["dt"][s]) <= 1e-12 * d["dt"][s]
+        v = sim.step(float(d["dt"][s]), float(d["t"][s]), float(d["rain"]))
+        for k, name in enumerate(("h", "hu", "hv", "s_filled", "infil_cum")):
+            assert_rel(_f64(getattr(sim, name)), st[s + 1][k], RTOL, ATOL, f"step {s} {name}")
+        got = np.array([float(v.bnd_in), float(v.bnd_out), float(v.rain), float(v.infil), float(v.clip)])
+        assert_rel(got, d["vols"][s], RTOL, 1e-9, f"step {s} volumes")
+
+
+@pytest.mark.parametrize("name", ["swe1d_net60", "swe1d_chain"])
+def test_swe1d_step_matches_reference(golden, gpu, name):
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    from his_amd.domain.channels import apply_gamma
+    from his_amd.solver.swe1d import SWE1D
+    d = golden(name)
+    B = int(d["B"])
+    ter = W.make_terrain(100, 81, 80.0)
+    cfg = SolverConfig(slot_width_frac=float(d["slot"]))
+    if B:
+        net = W.network60(ter)
+        one = SWE1D([apply_gamma(net, g) for g in d["gammas"]], cfg, gpu,
+                    inflow=W.triangle(0.0, 3600.0, [150.0, 320.0], base=10.0),
+                    stage=W.sine_tide(0.8, W.TIDE_PERIOD_S, [0.7, 1.9]))
+    else:
+        one = SWE1D(W.chain_network(ter, 20), cfg, gpu,
+                    inflow=W.triangle(0.0, 3600.0, [150.0], base=10.0, batched=False),
+                    stage=W.sine_tide(0.8, W.TIDE_PERIOD_S, [0.7], batched=False))
+    assert np.array_equal(one.topo.kind, d["kind"]) and np.array_equal(one.topo.face_l, d["fl"])
+    assert np.array_equal(one.topo.face_r, d["fr"])
+    for nm in ("sec_b", "sec_yb", "sec_ts", "fsec_b", "fsec_yb", "fsec_ts"):
+        sec = one.sec if nm.startswith("sec") else one.fsec
+        assert_rel(_f64(getattr(sec, nm.split("_")[1])), d[nm], 0, 0, nm)
+    st = d["states"]
+    for s in range(3):
+        one.A, one.Q = (torch.as_tensor(st[s][k], device=gpu) for k in range(2))
+        assert abs(one.max_dt() - d["max_dt"][s]) <= 1e-12 * d["max_dt"][s]
+        v = one.step(float(d["dt"][s]), float(d["t"][s]))
+        assert_rel(_f64(one.A), st[s + 1][0], RTOL, ATOL, f"step {s} A")
+        assert_rel(_f64(one.Q), st[s + 1][1], RTOL, ATOL, f"step {s} Q")
+        got = np.stack([_f64(v.bnd_in), _f64(v.bnd_out), _f64(v.clip)]).reshape(d["vols"][s].shape)
+        assert_rel(got, d["vols"][s], RTOL, 1e-9, f"step {s} volumes")
+
+
+def coupled_case(device, dtype_precision="fp64", B=2):
+    """The golden coupled workload: 100x81 @ 80 m, 20-reach chain, two members."""
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    from his_amd.engine import build_from_workload
+    wl = W.plain_cpu()
+    wl.members = [wl.members[0], W.Member(manning=0.05, retention_m=0.008, f0_mmh=40.0, gamma=1.2,
+                                          rain_peak_mmh=110.0, tide_phase=1.3, inflow_peak=220.0)][:B]
+    wl.precision = dtype_precision
+    eng, fo = build_from_workload(wl, SolverConfig(), device, batched=True)
+    return eng, fo
+
+
+def test_exchange_matches_reference(golden, gpu):
+    from his_amd.solver.coupling import exchange
+    d = golden("exchange")
+    eng, _ = coupled_case(gpu)
+    one, two = eng.one, eng.two
+    assert np.array_equal(_f64(eng.links.cell1d).astype(int), d["cell1d"])
+    assert np.array_equal(_f64(eng.links.inv).astype(int), d["inv"])
+    assert np.array_equal(_f64(eng.links.uniq_dev).astype(int), d["uniq"])
+    for s in range(3):
+        two.set_state(torch.as_tensor(d["h0"][s]), torch.as_tensor(d["hu0"][s]), torch.as_tensor(d["hv0"][s]))
+        one.A = torch.as_tensor(d["A0"][s], device=gpu).clone()
+        V = exchange(one, two, eng.links, float(d["dt"]), eng.cfg)
+        assert_rel(_f64(V), d["V"][s], RTOL, 1e-12, f"draw {s} V")
+        assert_rel(_f64(two.h), d["h1"][s], RTOL, ATOL, f"draw {s} h")
+        assert_rel(_f64(two.hu), d["hu1"][s], RTOL, ATOL, f"draw {s} hu")
+        assert_rel(_f64(two.hv), d["hv1"][s], RTOL, ATOL, f"draw {s} hv")
+        assert_rel(_f64(one.A), d["A1"][s], RTOL, ATOL, f"draw {s} A")
+
+
+def test_coupled_steps_match_reference(golden, gpu):
+    """40 coupled steps chosen by the engine itself: same dt / n_sub sequence, same state."""
+    d = golden("coupled")
+    eng, _ = coupled_case(gpu)
+    assert_rel(_f64(eng.two.h), d["init_h"], 0, 0, "initial h")
+    assert_rel(_f64(eng.one.A), d["init_A"], RTOL, ATOL, "initial A")
+    t, dts, nsubs = 0.0, [], []
+    for s in range(40):
+        dt = eng.two.max_dt()
+        n_sub = max(1, math.ceil(dt / eng.one.max_dt() - 1e-9))
+        t = eng.step(t, 1e9, dt=dt, n_sub=n_sub)
+        dts.append(dt)
+        nsubs.append(n_sub)
+    assert_rel(np.array(dts), d["dt"], RTOL, 0, "dt sequence")
+    assert nsubs == list(d["n_sub"])
+    for name, key in (("h", "h"), ("hu", "hu"), ("hv", "hv"), ("s_filled", "s"), ("infil_cum", "ic")):
+        assert_rel(_f64(getattr(eng.two, name)), d[key], 1e-8, 1e-12, f"2-D {name} after 40 steps")
+    assert_rel(_f64(eng.one.A), d["A"], 1e-8, 1e-12, "A after 40 steps")
+    assert_rel(_f64(eng.one.Q), d["Q"], 1e-8, 1e-10, "Q after 40 steps")
+    src = eng.mb.sources()
+    for k in ("rain", "infiltration", "bnd2d_in", "bnd2d_out", "bnd1d_in", "bnd1d_out"):
+        assert_rel(np.asarray(src[k]), d[f"src_{k}"], 1e-9, 1e-6, f"ledger {k}")
+
+
+def test_device_engine_matches_host_driven_steps(gpu):
+    """The graph-captured device loop takes the same steps as the host-driven per-op loop."""
+    eng_a, _ = coupled_case(gpu)
+    eng_b, _ = coupled_case(gpu)
+    t = 0.0
+    for s in range(60):
+        t = eng_a.step(t, 1e9)
+    eng_b.run(t, out_times=[])
+    assert eng_b.log.steps == 60
+    for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
+        assert_rel(_f64(getattr(eng_b.two, name)), _f64(getattr(eng_a.two, name)), 1e-12, 1e-14, f"2-D {name}")
+    assert_rel(_f64(eng_b.one.A), _f64(eng_a.one.A), 1e-12, 1e-14, "A")
+    assert_rel(_f64(eng_b.one.Q), _f64(eng_a.one.Q), 1e-12, 1e-12, "Q")
+    sa, sb = eng_a.mb.sources(), eng_b.mb.sources()
+    for k in sa:
+        assert_rel(np.asarray(sb[k]), np.asarray(sa[k]), 1e-12, 1e-9, f"ledger {k}")
+
+
+def test_device_engine_matches_oracle(gpu):
+    """Device engine vs the float64 CPU oracle running the same coupled order, 30 steps."""
+    from oracle.oracle import Oracle2D, OracleCoupled, OracleNet, oracle_links
+    from his_amd import workloads as W
+    from his_amd.domain.channels import apply_gamma
+    eng, fo = coupled_case(gpu)
+    wl = W.plain_cpu()
+    members = [wl.members[0], W.Member(manning=0.05, retention_m=0.008, f0_mmh=40.0, gamma=1.2,
+                                       rain_peak_mmh=110.0, tide_phase=1.3, inflow_peak=220.0)]
+    wl.members = members
+    n, S, f0, fc, k = wl.fields()
+    ter = wl.terrain
+    two = Oracle2D(ter.z, n, ter.dx, dict(west="transmissive", east="stage", south="transmissive",
+                                          north="transmissive"), B=2, storage=S, horton=(f0, fc, k, 0.0))
+    one = OracleNet([apply_gamma(wl.network, m.gamma) for m in members])
+    orc = OracleCoupled(two, one, oracle_links(one, ter.z, ter.dx), dict(east=fo.tide), fo.rain, fo.inflow,
+                        fo.tide)
+    orc.h = _f64(eng.two.h).copy()
+    orc.A = _f64(eng.one.A).copy()
+    t = 0.0
+    for s in range(30):
+        t = orc.step(t, 1e9)
+    eng.run(t, out_times=[])
+    assert eng.log.steps == 30
+    assert_rel(_f64(eng.two.h), orc.h, 1e-9, 1e-13, "h")
+    assert_rel(_f64(eng.two.hu), orc.hu, 1e-9, 1e-13, "hu")
+    assert_rel(_f64(eng.one.A), orc.A, 1e-9, 1e-13, "A")
+    assert_rel(_f64(eng.one.Q), orc.Q, 1e-9, 1e-11, "Q")
+
+
+def test_lake_at_rest_fp64(gpu):
+    """Well-balancedness (benchmarks/suite.py:84-116): 500 steps, closed box, partly dry DEM."""
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import all_edges
+    from his_amd.solver.swe2d import SWE2D
+    ter = W.make_terrain(100, 81, 80.0)
+    sim = SWE2D(ter.z, 0.035, ter.dx, SolverConfig(), all_edges("closed"), gpu, torch.float64)
+    sim.set_level(3.0)
+    wet0 = (sim.h > 0)
+    assert 0.1 < float(wet0.double().mean()) < 0.9
+    v0 = float(sim.volume())
+    t = 0.0
+    for _ in range(500):
+        dt = sim.max_dt()
+        sim.step(dt, t, 0.0)
+        t += dt
+    u, v = sim.velocities(sim.h, sim.hu, sim.hv)
+    assert float(u.abs().max()) < 1e-10 and float(v.abs().max()) < 1e-10
+    assert float(((sim.h + sim.z - 3.0).abs() * wet0).max()) < 1e-10
+    assert abs(float(sim.volume()) - v0) <= 1e-12 * v0
+
+
+def test_infiltration_capped_by_available_water(gpu):
+    """tests/test_massbalance.py:61-69 of the reference."""
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import all_edges
+    from his_amd.solver.infiltration import Horton
+    from his_amd.solver.swe2d import SWE2D
+    z = np.zeros((6, 6))
+    h = Horton.constant(1.0, z.shape, gpu, torch.float64)
+    sim = SWE2D(z, 0.03, 10.0, SolverConfig(), all_edges("closed"), gpu, torch.float64, horton=h)
+    sim.set_state(np.full(z.shape, 0.05))
+    vol = sim.step(1.0, 0.0, 0.0)
+    assert float(sim.h.min()) >= 0.0
+    assert abs(float(vol.infil) - 0.05 * 36 * 100.0) < 1e-9
+
+
+def test_storage_fills_before_runoff_and_never_changes_bed(gpu):
+    """tests/test_massbalance.py:72-86 of the reference."""
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import all_edges
+    from his_amd.solver.swe2d import SWE2D
+    z = np.zeros((5, 5))
+    sim = SWE2D(z, 0.03, 10.0, SolverConfig(), all_edges("closed"), gpu, torch.float64, storage=np.full(z.shape, 0.02))
+    z0 = sim.z.clone()
+    for k in range(10):
+        sim.step(1.0, float(k), 1e-3)
+    assert float(sim.h.max()) == 0.0
+    assert torch.allclose(sim.s_filled, torch.full_like(sim.s_filled, 0.01))
+    for k in range(20):
+        sim.step(1.0, float(k), 1e-3)
+    assert float(sim.h.min()) > 0.0
+    assert torch.equal(sim.z, z0)
+
+
+@pytest.mark.parametrize("store,h_end,s_end", [(0.03, 0.02, 0.03), (0.2, 0.0, 0.05)])
+def test_retained_water_closed_box(gpu, store, h_end, s_end):
+    """tests/test_retained_water.py:45-70 of the reference: 50 mm/h for 1 h on an 8x8 closed box."""
+    from his_amd.config import SolverConfig
+    from his_amd.solver.boundaries import EdgeBC
+    from his_amd.solver.massbalance import MassBalance
+    from his_amd.solver.swe2d import SWE2D
+    bcs = {e: EdgeBC("closed") for e in ("west", "east", "south", "north")}
+    sim = SWE2D(np.zeros((8, 8)), 0.03, 10.0, SolverConfig(), bcs, gpu, torch.float64, storage=np.full((8, 8), store))
+    rain = 50.0 / 3.6e6
+    mb = MassBalance(gpu, 0.0)
+    t = 0.0
+    while t < 3600.0 - 1e-9:
+        dt = min(sim.max_dt(), 30.0, 3600.0 - t)
+        mb.add("rain", sim.step(dt, t, rain).rain)
+        t += dt
+    assert torch.allclose(sim.s_filled, torch.full_like(sim.s_filled, s_end), atol=1e-12)
+    assert torch.allclose(sim.h, torch.full_like(sim.h, h_end), atol=1e-12)
+    assert abs(mb.sources()["rain"] - rain * 3600.0 * 64 * 100.0) < 1e-9 * rain * 3600.0 * 64 * 100.0
+    assert mb.check(float(sim.volume() + sim.storage_volume()), t) < 1e-12
+
+
+def test_fp32_bulk_step_tracks_fp64(golden, gpu):
+    """One step in float32 bulk against the float64 reference output: within float32 round-off."""
+    d = golden("swe2d_mixed")
+    sim = _swe2d_mixed(d, gpu, torch.float32)
+    st = d["states"]
+    ins = [torch.as_tensor(st[0][k], dtype=torch.float32, device=gpu) for k in range(5)]
+    out = sim._core(*ins, torch.tensor([d["dt"][0], d["t"][0]]), torch.as_tensor(d["pm"][0]).reshape(9, -1, 1, 1))
+    h, ref = _f64(out[0]), st[1][0]
+    assert np.abs(h - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
+    vols = np.stack([_f64(v) for v in out[5:]])
+    assert np.all(np.abs(vols - d["vols"][0]) <= 1e-4 * np.abs(d["vols"][0]) + 1e-3)
