@@ -38,6 +38,11 @@ def engine_throughput(wl, steps: int, warmup: int):
     from his_amd.engine import build_from_workload
     dev = torch.device("cuda")
     eng, fo = build_from_workload(wl, device=dev)
+    from his_amd import tuning
+    shape = tuning.tune(eng.two, force=True)
+    log(f"[bench]   kernel shape {shape} ({tuning.describe(eng.two.dtype, shape)}): "
+        + ", ".join(f"{s}:{t:.0f}us" for s, t in tuning.time_shapes(eng.two, 10).items()))
+    tuning.tune(eng.two)
     de = eng._device()
     de.t.fill_(0.0)
     de.t_end.fill_(1e12)

@@ -279,124 +279,194 @@ size_t swe1d_ws_doubles(const his_net1d& n) { return (size_t)n.B * (3 * (size_t)
 
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
                  hipStream_t s) {
-    hipLaunchKernelGGL(k_swe1d<256>, dim3(nblocks), dim3(256), 0, s, n, A, Q, d, ws);
+    hipLaunchKernelGGL(k_swe1d<1024>, dim3(nblocks), dim3(1024), 0, s, n, A, Q, d, ws);
     HIS_HIP(hipGetLastError());
     return 0;
 }
 
 int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
                       int member0, hipStream_t s) {
-    hipLaunchKernelGGL(k_swe1d_rate<256>, dim3(n.B), dim3(256), 0, s, n, A, Q, rate, wet, member0);
+    hipLaunchKernelGGL(k_swe1d_rate<1024>, dim3(n.B), dim3(1024), 0, s, n, A, Q, rate, wet, member0);
     HIS_HIP(hipGetLastError());
     return 0;
 }
 
 // ------------------------------------------------------------------ exchange
+//
+// Four grid-wide kernels, every member and link / unique 2-D cell / 1-D cell in flight at once:
+//   1. per link: free-weir + Villemonte volume capped by the level-equalising volume, its sign;
+//      per unique 2-D cell: the donor depth h_u
+//   2. per unique 2-D cell / 1-D cell: the donor limit factor from the links' gross give
+//   3. per unique 2-D cell: net volume -> new depth, momentum kept by min(h_new/h, 1);
+//      per 1-D cell: net volume -> new area   (sums run over each target's links in link order)
+//   4. per member: gross volumes each way into the ledger (fixed-order block reduction)
 
 size_t exchange_ws_doubles(const his_links& l, const his_net1d& n) {
-    return (size_t)n.B * (2 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc);
+    return (size_t)n.B * (3 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc);
 }
 
-template <typename R2, int NT>
-__global__ void __launch_bounds__(NT) k_exchange(const ExchangeArgs a) {
-    __shared__ double red[NT / 64][8];
+struct ExWs {
+    double *V, *sg, *Vf, *hu_, *f2, *f1;
+};
+
+__device__ __forceinline__ ExWs ex_ws(const ExchangeArgs& a, int b) {
+    const int nl = a.L.nl, nu = a.L.nu, nc = a.N.nc;
+    double* w = a.ws + (int64_t)b * (3 * nl + 2 * nu + nc);
+    return ExWs{w, w + nl, w + 2 * nl, w + 3 * nl, w + 3 * nl + nu, w + 3 * nl + 2 * nu};
+}
+
+template <typename R2>
+__global__ void __launch_bounds__(256) k_ex_links(const ExchangeArgs a) {
     const his_links& L = a.L;
     const his_net1d& N = a.N;
-    const int b = blockIdx.x;
+    const int per = L.nl + L.nu;
+    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= (int64_t)N.B * per) return;
+    const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     if (a.active && !a.active[m]) return;
-    const int nl = L.nl, nu = L.nu, nc = N.nc;
-    R2* h = reinterpret_cast<R2*>(a.h) + (int64_t)b * a.plane;
-    R2* hu = reinterpret_cast<R2*>(a.hu) + (int64_t)b * a.plane;
-    R2* hv = reinterpret_cast<R2*>(a.hv) + (int64_t)b * a.plane;
-    double* A = a.A + (int64_t)b * nc;
-    double* ws = a.ws + (int64_t)b * (2 * nl + 2 * nu + nc);
-    double *V = ws, *sg = ws + nl, *hu_ = ws + 2 * nl, *f2 = hu_ + nu, *f1 = f2 + nu;
-    const double dt = a.dt[m];
-    for (int k = threadIdx.x; k < nu; k += NT) hu_[k] = (double)h[L.uniq[k]];
-    __syncthreads();
-    for (int l = threadIdx.x; l < nl; l += NT) {
-        const int c1 = L.cell1d[l];
-        const Sec s = cell_sec(N, b, c1);
-        const double h2 = hu_[L.inv[l]];
-        const double eta2 = L.z2[l] + h2;
-        const double y1 = sec_depth(s, A[c1]);
-        const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
-        const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
-        const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
-        const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
-        const double q = L.weir_cw * L.length[l] * pow(d_up, 1.5) * pow(clamp0(1.0 - pow(ratio, 1.5)), 0.385);
-        sg[l] = eta2 > eta1 ? 1.0 : -1.0;
-        const double v = q * dt;
-        const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
-        const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
-        V[l] = tmin(v, L.relax * v_eq);
+    const R2* h = reinterpret_cast<const R2*>(a.h) + (int64_t)b * a.plane;
+    ExWs w = ex_ws(a, b);
+    if (r >= L.nl) {
+        const int k = r - L.nl;
+        w.hu_[k] = (double)h[L.uniq[k]];
+        return;
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nu; k += NT) {
+    const int l = r;
+    const double dt = a.dt[m];
+    const int c1 = L.cell1d[l];
+    const Sec s = cell_sec(N, b, c1);
+    const double h2 = (double)h[L.uniq[L.inv[l]]];
+    const double eta2 = L.z2[l] + h2;
+    const double y1 = sec_depth(s, a.A[(int64_t)b * N.nc + c1]);
+    const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
+    const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
+    const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
+    const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
+    const double q = L.weir_cw * L.length[l] * pow(d_up, 1.5) * pow(clamp0(1.0 - pow(ratio, 1.5)), 0.385);
+    w.sg[l] = eta2 > eta1 ? 1.0 : -1.0;
+    const double v = q * dt;
+    const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
+    const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
+    w.V[l] = tmin(v, L.relax * v_eq);
+}
+
+__global__ void __launch_bounds__(256) k_ex_limits(const ExchangeArgs a) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    const int per = L.nu + N.nc;
+    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= (int64_t)N.B * per) return;
+    const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
+    const int m = a.member0 + b;
+    if (a.active && !a.active[m]) return;
+    ExWs w = ex_ws(a, b);
+    if (r < L.nu) {
+        const int k = r;
         double tot = 0.0;
         for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
             const int l = L.u_link[p];
-            tot += sg[l] > 0.0 ? V[l] : 0.0;
+            tot += w.sg[l] > 0.0 ? w.V[l] : 0.0;
         }
-        const double av = L.limit * hu_[k] * L.area2;
-        f2[k] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+        const double av = L.limit * w.hu_[k] * L.area2;
+        w.f2[k] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+        return;
     }
-    for (int c = threadIdx.x; c < nc; c += NT) {
-        double tot = 0.0;
-        for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
-            const int l = L.c_link[p];
-            tot += sg[l] < 0.0 ? V[l] : 0.0;
-        }
-        const Sec s = cell_sec(N, b, c);
-        const double av = L.limit * clamp0(A[c] - bank_area(s)) * N.dx[c];
-        f1[c] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+    const int c = r - L.nu;
+    double tot = 0.0;
+    for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
+        const int l = L.c_link[p];
+        tot += w.sg[l] < 0.0 ? w.V[l] : 0.0;
     }
-    __syncthreads();
-    double part[2] = {0.0, 0.0};
-    for (int l = threadIdx.x; l < nl; l += NT) {
-        const double w = sg[l] > 0.0 ? V[l] * f2[L.inv[l]] : V[l] * f1[L.cell1d[l]];
-        const double v = w * sg[l];
-        V[l] = v;
-        if (a.V) a.V[(int64_t)b * nl + l] = v;
-        part[0] += clamp0(v);
-        part[1] += clamp0(-v);
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < nu; k += NT) {
+    const Sec s = cell_sec(N, b, c);
+    const double av = L.limit * clamp0(a.A[(int64_t)b * N.nc + c] - bank_area(s)) * N.dx[c];
+    w.f1[c] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+}
+
+__device__ __forceinline__ double ex_final(const ExWs& w, const his_links& L, int l) {
+    const double v = w.sg[l] > 0.0 ? w.V[l] * w.f2[L.inv[l]] : w.V[l] * w.f1[L.cell1d[l]];
+    return v * w.sg[l];
+}
+
+template <typename R2>
+__global__ void __launch_bounds__(256) k_ex_apply(const ExchangeArgs a) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    const int per = L.nu + N.nc;
+    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= (int64_t)N.B * per) return;
+    const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
+    const int m = a.member0 + b;
+    if (a.active && !a.active[m]) return;
+    ExWs w = ex_ws(a, b);
+    if (r < L.nu) {
+        const int k = r;
         double dV = 0.0;
-        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) dV += V[L.u_link[p]];
-        const double h0 = hu_[k];
+        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
+            const int l = L.u_link[p];
+            const double v = ex_final(w, L, l);
+            w.Vf[l] = v;
+            if (a.V) a.V[(int64_t)b * L.nl + l] = v;
+            dV += v;
+        }
+        const double h0 = w.hu_[k];
         const double hn = h0 - dV / L.area2;
         const double keep = h0 > 0.0 ? tmin(hn / (h0 > 0.0 ? h0 : 1.0), 1.0) : 0.0;
-        const int64_t q = L.uniq[k];
+        const int64_t q = (int64_t)b * a.plane + L.uniq[k];
+        R2* h = reinterpret_cast<R2*>(a.h);
+        R2* hu = reinterpret_cast<R2*>(a.hu);
+        R2* hv = reinterpret_cast<R2*>(a.hv);
         const R2 kp = (R2)keep;
         h[q] = (R2)hn;
         hu[q] = hu[q] * kp;
         hv[q] = hv[q] * kp;
+        return;
     }
-    for (int c = threadIdx.x; c < nc; c += NT) {
-        const int p0 = L.c_ptr[c], p1 = L.c_ptr[c + 1];
-        if (p0 == p1) continue;
-        double add = 0.0;
-        for (int p = p0; p < p1; ++p) add += V[L.c_link[p]];
-        A[c] = A[c] + add / N.dx[c];
+    const int c = r - L.nu;
+    const int p0 = L.c_ptr[c], p1 = L.c_ptr[c + 1];
+    if (p0 == p1) return;
+    double add = 0.0;
+    for (int p = p0; p < p1; ++p) add += ex_final(w, L, L.c_link[p]);
+    double* A = a.A + (int64_t)b * N.nc;
+    A[c] = A[c] + add / N.dx[c];
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_ex_ledger(const ExchangeArgs a) {
+    __shared__ double red[NT / 64][8];
+    const int b = blockIdx.x;
+    const int m = a.member0 + b;
+    if (a.active && !a.active[m]) return;
+    ExWs w = ex_ws(a, b);
+    double part[2] = {0.0, 0.0};
+    for (int l = threadIdx.x; l < a.L.nl; l += NT) {
+        const double v = w.Vf[l];
+        part[0] += clamp0(v);
+        part[1] += clamp0(-v);
     }
-    if (a.acc) {
-        block_sum_bcast<NT, 2>(part, red);
-        if (threadIdx.x == 0) {
-            a.acc[(int64_t)a.acc_slot_in * a.acc_stride + m] += part[0];
-            a.acc[(int64_t)(a.acc_slot_in + 1) * a.acc_stride + m] += part[1];
-        }
+    block_sum_bcast<NT, 2>(part, red);
+    if (threadIdx.x == 0) {
+        a.acc[(int64_t)a.acc_slot_in * a.acc_stride + m] += part[0];
+        a.acc[(int64_t)(a.acc_slot_in + 1) * a.acc_stride + m] += part[1];
     }
 }
 
 int launch_exchange(const ExchangeArgs& a, hipStream_t s) {
-    if (a.dtype2d == HIS_F64)
-        hipLaunchKernelGGL((k_exchange<double, 256>), dim3(a.N.B), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_exchange<float, 256>), dim3(a.N.B), dim3(256), 0, s, a);
+    const int B = a.N.B;
+    auto blocks = [](int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); };
+    const dim3 g1 = blocks((int64_t)B * (a.L.nl + a.L.nu)), g2 = blocks((int64_t)B * (a.L.nu + a.N.nc));
+    if (a.dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, a);
     HIS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ex_limits, g2, dim3(256), 0, s, a);
+    HIS_HIP(hipGetLastError());
+    if (a.dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, a);
+    HIS_HIP(hipGetLastError());
+    if (a.acc) {
+        hipLaunchKernelGGL(k_ex_ledger<256>, dim3(B), dim3(256), 0, s, a);
+        HIS_HIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -41,11 +41,16 @@ struct Launch2D {
     double *part;                 // (ntiles, NP2)
 };
 
-// tile shapes compiled for each precision; index into this table is the tuning knob
-struct TileShape { int tx, ty, nt; };
-constexpr TileShape kShapesF32[] = {{64, 8, 256}, {64, 4, 256}, {64, 16, 256}, {32, 8, 256}};
-constexpr TileShape kShapesF64[] = {{32, 8, 256}, {64, 4, 256}, {64, 8, 256}, {32, 4, 128}};
-constexpr int kNumShapes = 4;
+// Kernel shapes compiled for each precision; the index into this table is the tuning knob.
+//   march = 0: a TX x TY tile per workgroup of NT threads, halo staged in LDS (k_stage)
+//   march = 1: one wave marches up a strip of TX = 58 columns for TY rows with LDS row rings (k_march)
+//   march = 2: the same walk with each lane's column held in registers, x-neighbours by shuffle (k_wave)
+struct TileShape { int tx, ty, nt, march; };
+constexpr TileShape kShapesF32[] = {{58, 16, 64, 2}, {58, 32, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
+                                    {64, 8, 256, 0}, {64, 16, 256, 0}};
+constexpr TileShape kShapesF64[] = {{58, 16, 64, 2}, {58, 32, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
+                                    {32, 8, 256, 0}, {64, 4, 256, 0}};
+constexpr int kNumShapes = 6;
 
 // host launchers, defined once per precision (swe2d_f32.hip, swe2d_f64.hip)
 int launch_step(const Launch2D<float>& L, int shape, hipStream_t s);

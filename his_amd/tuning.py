@@ -1,0 +1,87 @@
+"""Autotuner for the 2-D kernel shape.
+
+For a grid (dtype, nx, ny, B) it times every compiled kernel shape -- LDS-tiled
+workgroups of several tile sizes and column-marching waves of several chunk
+heights -- on the fused 2-D step with HIP events, and keeps the fastest.  Results
+persist as JSON keyed by "dtype:nx:ny:B" in $HIS_AMD_TUNING (default
+~/.cache/his_amd/tuning.json) so a later process reuses them without timing.
+"""
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+N_SHAPES = 6
+
+
+def _path() -> Path:
+    return Path(os.environ.get("HIS_AMD_TUNING", Path.home() / ".cache" / "his_amd" / "tuning.json"))
+
+
+def _load() -> dict:
+    p = _path()
+    try:
+        return json.loads(p.read_text())
+    except Exception:
+        return {}
+
+
+def _save(tab: dict) -> None:
+    p = _path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(tab, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def key(dtype, nx, ny, B) -> str:
+    return f"{'f64' if dtype == torch.float64 else 'f32'}:{nx}:{ny}:{B}"
+
+
+def time_shapes(two, reps: int = 20) -> dict:
+    """Mean microseconds of the 2-D step for every shape, on a copy of ``two``'s state."""
+    two._sync_state()
+    st = (two.h.clone(), two.hu.clone(), two.hv.clone(), two.s_filled.clone(), two.infil_cum.clone())
+    pm = two._pack([0.0] * 9)
+    dt_t = two._pack([0.05, 0.0])
+    out = {}
+    for shape in range(N_SHAPES):
+        L.set_tile_shape(two.dtype, shape)
+        for _ in range(3):
+            two._run(*st, dt_t, pm)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            two._run(*st, dt_t, pm)
+        e1.record()
+        torch.cuda.synchronize()
+        out[shape] = 1e3 * e0.elapsed_time(e1) / reps
+    return out
+
+
+def tune(two, reps: int = 20, force: bool = False) -> int:
+    """Select (and remember) the fastest shape for this solver's grid; sets it active."""
+    k = key(two.dtype, two.nx, two.ny, two.B)
+    tab = _load()
+    if not force and k in tab:
+        best = int(tab[k]["shape"])
+    else:
+        times = time_shapes(two, reps)
+        best = int(min(times, key=times.get))
+        tab[k] = {"shape": best, "us": times[best], "all_us": {str(s): t for s, t in times.items()}}
+        _save(tab)
+    L.set_tile_shape(two.dtype, best)
+    return best
+
+
+def describe(dtype, shape: int) -> str:
+    tx, ty = L.tile_shape(dtype, shape)
+    return f"march {tx}x{ty}" if tx == 58 else f"tile {tx}x{ty}"

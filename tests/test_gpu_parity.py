@@ -306,3 +306,30 @@ def test_fp32_bulk_step_tracks_fp64(golden, gpu):
     assert np.abs(h - ref).max() < 1e-5 * max(1.0, np.abs(ref).max())
     vols = np.stack([_f64(v) for v in out[5:]])
     assert np.all(np.abs(vols - d["vols"][0]) <= 1e-4 * np.abs(d["vols"][0]) + 1e-3)
+
+
+@pytest.mark.parametrize("shape", range(6))
+def test_every_kernel_shape_matches_reference(golden, gpu, shape):
+    """Tiled and marching kernel variants (the tuner's choices) all reproduce the reference step."""
+    from his_amd import _lib
+    d = golden("swe2d_mixed")
+    st = d["states"]
+    try:
+        for dtype, rtol in ((torch.float64, RTOL), (torch.float32, None)):
+            _lib.set_tile_shape(dtype, shape)
+            sim = _swe2d_mixed(d, gpu, dtype)
+            for s in range(2):
+                ins = [torch.as_tensor(st[s][k], dtype=dtype, device=gpu) for k in range(5)]
+                sc = torch.tensor([d["dt"][s], d["t"][s]], dtype=torch.float64)
+                out = sim._core(*ins, sc, torch.as_tensor(d["pm"][s]).reshape(9, -1, 1, 1))
+                if rtol is not None:
+                    for k in range(5):
+                        assert_rel(_f64(out[k]), st[s + 1][k], rtol, ATOL, f"shape {shape} step {s} field {k}")
+                    assert_rel(np.stack([_f64(v) for v in out[5:]]), d["vols"][s], RTOL, 1e-9, f"shape {shape} vols")
+                else:
+                    assert np.abs(_f64(out[0]) - st[s + 1][0]).max() < 1e-5
+            rates = _f64(sim.member_rates(*(torch.as_tensor(st[0][k], dtype=dtype, device=gpu) for k in range(3))))
+            assert_rel(rates, d["rate_member_0"], RTOL if dtype == torch.float64 else 1e-5, 0, f"shape {shape} rate")
+    finally:
+        _lib.set_tile_shape(torch.float64, 0)
+        _lib.set_tile_shape(torch.float32, 0)
