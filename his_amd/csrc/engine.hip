@@ -119,6 +119,10 @@ __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
         F.vols[3 * F.Btot + m] = inf_v;
         F.vols[4 * F.Btot + m] = clip;
     }
+    if (F.t_out && F.advance[gi]) {
+        F.t_out[m] = F.t_step[m] + dt;
+        F.steps[m] += 1;
+    }
     if (F.acc) {
         F.acc[0 * F.Btot + m] += rain_v;
         F.acc[1 * F.Btot + m] += inf_v;
@@ -410,6 +414,7 @@ struct DtArgs {
     double cfl_1d[HIS_MAX_GROUPS], dt_max_1d[HIS_MAX_GROUPS];
     his_control ctl;
     int32_t* active;
+    double* t_step;            // (Btot) start time of the current step
     double* pm;                // (9, Btot)
     const double* rate2d;      // (Btot)
     const double* rate1d;      // (Btot)
@@ -418,79 +423,86 @@ struct DtArgs {
 };
 
 // One block per shared-dt group: choose dt and n_sub, mark members active, evaluate the forcing.
-__global__ void k_dt(const DtArgs a) {
+// Snapshots each member's step start time into t_step, which the step's kernels read while the
+// 1-D kernel (or finalize, without a network) advances t.
+__global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
     const DtGroup G = a.groups[blockIdx.x];
     const int gi = G.gi;
+    __shared__ double red[4][3];
     __shared__ double sh_dt;
-    __shared__ int sh_act;
+    __shared__ int sh_act, sh_nsub;
+    double r2 = 0.0, r1 = 0.0, w1 = 0.0;
+    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
+        r2 = a.rate2d[m] > r2 ? a.rate2d[m] : r2;
+        if (a.has_net[gi]) {
+            r1 = a.rate1d[m] > r1 ? a.rate1d[m] : r1;
+            w1 = a.wet1d[m] ? 1.0 : w1;
+        }
+    }
+    r2 = wave_max(r2);
+    r1 = wave_max(r1);
+    w1 = wave_max(w1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { red[w][0] = r2; red[w][1] = r1; red[w][2] = w1; }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            r2 = red[k][0] > r2 ? red[k][0] : r2;
+            r1 = red[k][1] > r1 ? red[k][1] : r1;
+            w1 = red[k][2] > w1 ? red[k][2] : w1;
+        }
         const int m0 = G.m0;
         const double t = a.ctl.t[m0];
         const double lim = a.ctl.t_limit[m0] < a.ctl.t_end[m0] ? a.ctl.t_limit[m0] : a.ctl.t_end[m0];
         const int act = t < lim - 1e-9;
         double dt = 0.0;
-        int nsub = 0;
+        int nsub = 0, clamp = 0;
         if (act) {
-            double r2 = 0.0, r1 = 0.0;
-            int w1 = 0;
-            for (int m = G.m0; m < G.m1; ++m) {
-                r2 = a.rate2d[m] > r2 ? a.rate2d[m] : r2;
-                if (a.has_net[gi]) {
-                    r1 = a.rate1d[m] > r1 ? a.rate1d[m] : r1;
-                    w1 |= a.wet1d[m];
-                }
-            }
             double d2 = r2 > 0.0 ? a.cfl / r2 : a.dt_max;
             if (d2 < a.dt_min || d2 > a.dt_max) {
-                if (d2 < a.dt_min)
-                    for (int m = G.m0; m < G.m1; ++m) a.ctl.clamped[m] += 1;
+                clamp = d2 < a.dt_min;
                 d2 = d2 < a.dt_min ? a.dt_min : d2;
                 d2 = d2 > a.dt_max ? a.dt_max : d2;
             }
             dt = d2 < lim - t ? d2 : lim - t;
             if (a.has_net[gi]) {
                 double d1 = a.dt_max_1d[gi];
-                if (w1) {
+                if (w1 > 0.0) {
                     d1 = a.cfl_1d[gi] / r1;
                     d1 = d1 < a.dt_max_1d[gi] ? d1 : a.dt_max_1d[gi];
                 }
-                double q = ceil(dt / d1 - 1e-9);
+                const double q = ceil(dt / d1 - 1e-9);
                 nsub = q < 1.0 ? 1 : (int)q;
-            }
-        }
-        for (int m = G.m0; m < G.m1; ++m) {
-            a.active[m] = act;
-            if (act) {
-                a.ctl.dt[m] = dt;
-                a.ctl.n_sub[m] = nsub;
-                a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
-                a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
-                a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
             }
         }
         sh_dt = dt;
         sh_act = act;
+        sh_nsub = nsub;
+        red[0][0] = clamp;
     }
     __syncthreads();
-    if (!sh_act) return;
+    const int act = sh_act;
     const double dt = sh_dt;
+    const int nsub = sh_nsub;
+    const int clamp = red[0][0] != 0.0;
     for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
-        const int b = m - a.member0[gi];
+        a.active[m] = act;
+        if (!act) continue;
         const double t = a.ctl.t[m];
+        a.t_step[m] = t;
+        a.ctl.dt[m] = dt;
+        a.ctl.n_sub[m] = nsub;
+        if (clamp) a.ctl.clamped[m] += 1;
+        a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
+        a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
+        a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
+        const int b = m - a.member0[gi];
         a.pm[m] = series_eval(a.rain[gi], b, t);
         for (int e = 0; e < 4; ++e) {
             const bool valued = a.bc[gi][e] == HIS_BC_STAGE || a.bc[gi][e] == HIS_BC_DISCHARGE;
             a.pm[(1 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t) : 0.0;
             a.pm[(5 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t + dt) : 0.0;
         }
-    }
-}
-
-__global__ void k_advance(his_control ctl, const int32_t* active, int Btot) {
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < Btot; m += gridDim.x * blockDim.x) {
-        if (!active[m]) continue;
-        ctl.t[m] = ctl.t[m] + ctl.dt[m];
-        ctl.steps[m] += 1;
     }
 }
 
@@ -510,7 +522,7 @@ struct his_engine {
     double* ws1d[HIS_MAX_GROUPS];
     double* wsx[HIS_MAX_GROUPS];
     int32_t* active;
-    double *pm, *rate2d, *rate1d, *volume;
+    double *pm, *rate2d, *rate1d, *volume, *t_step;
     int32_t* wet1d;
     hipStream_t cap, side;
     hipEvent_t ev_fork, ev_join;
@@ -529,7 +541,7 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
         ExchangeArgs a{};
         a.L = e->links[gi]; a.N = e->net[gi]; a.dtype2d = e->dtype; a.plane = (int64_t)G.grid.ny * G.grid.nx;
         a.h = G.state.h; a.hu = G.state.hu; a.hv = G.state.hv; a.A = G.A; a.dt = e->d.ctl.dt; a.V = nullptr;
-        a.acc = e->d.ctl.acc; a.acc_slot_in = 8; a.acc_stride = e->Btot; a.active = e->active; a.ws = e->wsx[gi];
+        a.acc = nullptr; a.acc_slot_in = 8; a.acc_stride = e->Btot; a.active = e->active; a.ws = e->wsx[gi];
         a.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
         if ((rc = launch_exchange(a, s))) return rc;
     }
@@ -545,7 +557,11 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
         if (!e->has_net[gi]) continue;
         const his_group& G = e->d.group[gi];
         Net1DDrive d{};
-        d.dt = e->d.ctl.dt; d.engine = 1; d.n_sub = e->d.ctl.n_sub; d.t = e->d.ctl.t;
+        d.dt = e->d.ctl.dt; d.engine = 1; d.n_sub = e->d.ctl.n_sub; d.t = e->t_step;
+        d.t_out = e->d.ctl.t; d.steps = e->d.ctl.steps;
+        d.ex_vf = exchange_vf(e->wsx[gi], e->links[gi], e->net[gi], 0);
+        d.ex_nl = e->links[gi].nl;
+        d.ex_stride = 3 * e->links[gi].nl + 2 * e->links[gi].nu + e->net[gi].nc;
         d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
         d.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
@@ -558,8 +574,6 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
         HIS_HIP(hipEventRecord(e->ev_join, side));
         HIS_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(256), 0, s, e->d.ctl, e->active, e->Btot);
-    HIS_HIP(hipGetLastError());
     return 0;
 }
 
@@ -668,20 +682,21 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     e->pm = (double*)p; p += align_up((size_t)Btot * 9 * 8);
     e->rate2d = (double*)p; p += align_up((size_t)Btot * 8);
     e->rate1d = (double*)p; p += align_up((size_t)Btot * 8);
-    p += align_up((size_t)Btot * 8);
+    e->t_step = (double*)p; p += align_up((size_t)Btot * 8);
     e->volume = (double*)p; p += align_up((size_t)Btot * 2 * 8);
     e->wet1d = (int32_t*)p; p += align_up((size_t)Btot * 4);
     DtGroup* dtg_dev = (DtGroup*)p;
     // launches
     if (dtype == HIS_F64) {
-        e->L64.dt = c.dt; e->L64.t = c.t; e->L64.pm = e->pm; e->L64.active = e->active; e->L64.part = part;
+        e->L64.dt = c.dt; e->L64.t = e->t_step; e->L64.pm = e->pm; e->L64.active = e->active; e->L64.part = part;
     } else {
-        e->L32.dt = c.dt; e->L32.t = c.t; e->L32.pm = e->pm; e->L32.active = e->active; e->L32.part = part;
+        e->L32.dt = c.dt; e->L32.t = e->t_step; e->L32.pm = e->pm; e->L32.active = e->active; e->L32.part = part;
     }
     FinalizeArgs& F = e->F;
     F = FinalizeArgs{};
     F.part = part; F.dt = c.dt; F.pm = e->pm; F.active = e->active; F.acc = c.acc; F.rate = e->rate2d;
     F.volume = e->volume; F.Btot = Btot; F.ngroups = d->ngroups;
+    F.t_out = c.t; F.t_step = e->t_step; F.steps = c.steps;
     std::vector<DtGroup> dtg;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];
@@ -690,13 +705,14 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
         const int per = dtype == HIS_F64 ? e->L64.g[gi].tiles_x * e->L64.g[gi].tiles_y
                                          : e->L32.g[gi].tiles_x * e->L32.g[gi].tiles_y;
         fill_finalize(F, gi, G.grid, m0, t0, per);
+        F.advance[gi] = G.net == nullptr;
         for (int k = 0; k < G.grid.B; k += G.dt_group_size) dtg.push_back(DtGroup{m0 + k, m0 + k + G.dt_group_size, gi, 0});
     }
     e->n_dtg = (int)dtg.size();
     DtArgs& D = e->D;
     D = DtArgs{};
     D.groups = dtg_dev; D.ngroups_dt = e->n_dtg; D.cfl = d->cfl; D.dt_min = d->dt_min; D.dt_max = d->dt_max;
-    D.ctl = c; D.active = e->active; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
+    D.ctl = c; D.active = e->active; D.t_step = e->t_step; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];

@@ -123,6 +123,21 @@ __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restr
         t0 = d.t[m];
     }
     const double sub = d.engine ? dt_full / nsub : dt_full;
+    if (d.engine && d.ex_vf) {
+        // gross exchanged volumes of this step's weir exchange (coupling.py:210-211), fixed order
+        const double* vf = d.ex_vf + (int64_t)b * d.ex_stride;
+        double part[2] = {0.0, 0.0};
+        for (int l = threadIdx.x; l < d.ex_nl; l += NT) {
+            part[0] += clamp0(vf[l]);
+            part[1] += clamp0(-vf[l]);
+        }
+        block_sum_bcast<NT, 2>(part, red);
+        if (threadIdx.x == 0) {
+            d.acc[8 * d.Btot + m] += part[0];
+            d.acc[9 * d.Btot + m] += part[1];
+        }
+        __syncthreads();
+    }
     for (int k = 0; k < nsub; ++k) {
         double q_in, eta;
         if (d.engine) {
@@ -152,7 +167,7 @@ __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restr
             if (wet_i && (un > ci)) { un_st = un; Ast = As; }
             if (!(sec_depth(s, Ast) >= h_dry)) un_st = 0.0;
             const double Q_st = sg * Ast * un_st;
-            double yc = pow(q_in * q_in / (g * s.b * s.b), 1.0 / 3.0);
+            double yc = cbrt(q_in * q_in / (g * s.b * s.b));
             if (yc < h_dry) yc = h_dry;
             const double A_in = wet_i ? As : sec_area(s, yc);
             const double Q_in = -sg * q_in;
@@ -243,7 +258,7 @@ __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restr
             const double Rh = Ac / sec_perim(s, y1);
             const double Rc = Rh < h_dry ? h_dry : Rh;
             const double nm = N.nman[c];
-            const double denom = 1.0 + dt * g * (nm * nm) * fabs(Q1) / (Ac * pow(Rc, 4.0 / 3.0));
+            const double denom = 1.0 + dt * g * (nm * nm) * fabs(Q1) / (Ac * (Rc * cbrt(Rc)));
             Q1 = Q1 / denom;
             if (!((y1 >= h_dry) && kd != 1)) Q1 = 0.0;
             A[c] = A1;
@@ -265,6 +280,10 @@ __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restr
         __syncthreads();
     }
     if (d.rate || d.wet) rate1d<NT>(N, b, A, Q, d.rate, d.wet, m, red);
+    if (d.engine && d.t_out && threadIdx.x == 0) {
+        d.t_out[m] = t0 + dt_full;
+        d.steps[m] += 1;
+    }
 }
 
 template <int NT>
@@ -343,7 +362,7 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExchangeArgs a) {
     const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
     const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
     const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
-    const double q = L.weir_cw * L.length[l] * pow(d_up, 1.5) * pow(clamp0(1.0 - pow(ratio, 1.5)), 0.385);
+    const double q = L.weir_cw * L.length[l] * (d_up * sqrt(d_up)) * pow(clamp0(1.0 - ratio * sqrt(ratio)), 0.385);
     w.sg[l] = eta2 > eta1 ? 1.0 : -1.0;
     const double v = q * dt;
     const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];

@@ -21,6 +21,12 @@ struct Net1DDrive {
     double *rate;
     int32_t *wet;
     int Btot, member0;
+    // engine mode: fold the exchange's signed link volumes (B, ex_stride) into acc slots 8 / 9 first,
+    // and advance t_out[m] = t[m] + dt[m], steps[m] += 1 at the end
+    const double *ex_vf;
+    int ex_nl, ex_stride;
+    double *t_out;
+    int32_t *steps;
 };
 
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
@@ -47,6 +53,10 @@ struct ExchangeArgs {
 int launch_exchange(const ExchangeArgs& a, hipStream_t s);
 
 size_t swe1d_ws_doubles(const his_net1d& n);
+// signed link volumes of the last exchange for member b (float64, nl values) inside its workspace
+inline const double* exchange_vf(double* ws, const his_links& l, const his_net1d& n, int b) {
+    return ws + (size_t)b * (3 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc) + 2 * (size_t)l.nl;
+}
 size_t exchange_ws_doubles(const his_links& l, const his_net1d& n);
 
 }  // namespace his

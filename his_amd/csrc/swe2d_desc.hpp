@@ -78,6 +78,11 @@ struct FinalizeArgs {
     double dx[HIS_MAX_GROUPS], dy[HIS_MAX_GROUPS], area[HIS_MAX_GROUPS], cells[HIS_MAX_GROUPS];
     int rain_f32[HIS_MAX_GROUPS];        // rain * dt formed in float32 (the working dtype) before widening
     int stage_only;                      // 1 = only rate/volume (after a state reduction)
+    // engine: members of groups without a 1-D network advance here, t_out = t_step + dt, steps += 1
+    double* t_out;
+    const double* t_step;
+    int32_t* steps;
+    int advance[HIS_MAX_GROUPS];
 };
 int launch_finalize(const FinalizeArgs& f, hipStream_t s);
 
