@@ -62,7 +62,8 @@ class Group(C.Structure):
 
 class Control(C.Structure):
     _fields_ = [("t", VP), ("t_limit", VP), ("t_end", VP), ("dt", VP), ("n_sub", VP), ("steps", VP),
-                ("clamped", VP), ("acc", VP), ("dt_min_seen", VP), ("dt_max_seen", VP), ("n_sub_max", VP)]
+                ("clamped", VP), ("acc", VP), ("dt_min_seen", VP), ("dt_max_seen", VP), ("n_sub_max", VP),
+                ("error", VP), ("n_sub_limit", C.c_int32)]
 
 
 class EngineDesc(C.Structure):
@@ -128,7 +129,9 @@ def require_gpu(device) -> torch.device:
         raise RuntimeError("his_amd computes on a ROCm GPU through libhis_amd.so; no GPU device is available "
                            f"(asked for {dev})")
     load()
-    return dev
+    # always carry an explicit index: torch.device("cuda") != torch.device("cuda:0"), and state tensors are
+    # compared by device before they are handed to the library as raw pointers
+    return torch.device("cuda", dev.index if dev.index is not None else torch.cuda.current_device())
 
 
 def stream(device) -> int:

@@ -456,8 +456,9 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
         const double lim = a.ctl.t_limit[m0] < a.ctl.t_end[m0] ? a.ctl.t_limit[m0] : a.ctl.t_end[m0];
         const int act = t < lim - 1e-9;
         double dt = 0.0;
-        int nsub = 0, clamp = 0;
-        if (act) {
+        int nsub = 0, clamp = 0, err = 0;
+        if (act && !(r2 == r2 && r2 < 1e300 && r1 == r1 && r1 < 1e300)) err = 2;   // NaN or inf rate
+        if (act && !err) {
             double d2 = r2 > 0.0 ? a.cfl / r2 : a.dt_max;
             if (d2 < a.dt_min || d2 > a.dt_max) {
                 clamp = d2 < a.dt_min;
@@ -472,11 +473,15 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
                     d1 = d1 < a.dt_max_1d[gi] ? d1 : a.dt_max_1d[gi];
                 }
                 const double q = ceil(dt / d1 - 1e-9);
-                nsub = q < 1.0 ? 1 : (int)q;
+                if (q > (double)a.ctl.n_sub_limit) err = 1;
+                else nsub = q < 1.0 ? 1 : (int)q;
             }
         }
+        if (err) {
+            for (int m = G.m0; m < G.m1; ++m) a.ctl.error[m] = err;
+        }
         sh_dt = dt;
-        sh_act = act;
+        sh_act = act && !err && a.ctl.error[m0] == 0;
         sh_nsub = nsub;
         red[0][0] = clamp;
     }
@@ -649,7 +654,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     }
     const his_control& c = d->ctl;
     if (!c.t || !c.t_limit || !c.t_end || !c.dt || !c.n_sub || !c.steps || !c.clamped || !c.acc || !c.dt_min_seen ||
-        !c.dt_max_seen || !c.n_sub_max) {
+        !c.dt_max_seen || !c.n_sub_max || !c.error || c.n_sub_limit < 1) {
         set_error("his_engine_create: control block has a NULL array");
         return HIS_E_ARG;
     }

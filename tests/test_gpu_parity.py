@@ -187,6 +187,29 @@ def test_device_engine_matches_host_driven_steps(gpu):
         assert_rel(np.asarray(sb[k]), np.asarray(sa[k]), 1e-12, 1e-9, f"ledger {k}")
 
 
+def test_device_engine_survives_snapshots_and_checks():
+    """Snapshots, plausibility and mass checks between device launches leave the engine's buffers bound:
+    the device loop, stopped at output times and checked every few steps, takes the host loop's steps.
+    The device is named without an index, as callers usually do."""
+    dev = torch.device("cuda")
+    eng_a, _ = coupled_case(dev)
+    eng_b, _ = coupled_case(dev)
+    outs = [4.0 * k for k in range(1, 16)]
+    t, k = 0.0, 0
+    for limit in outs:
+        while t < limit - 1e-9:
+            t = eng_a.step(t, limit)
+            k += 1
+    seen = []
+    eng_b.run(outs[-1], out_times=[0.0] + outs, on_snapshot=lambda s: seen.append(s.t), check_every=3)
+    assert eng_b.log.steps == k
+    assert seen == [0.0] + outs
+    for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
+        assert_rel(_f64(getattr(eng_b.two, name)), _f64(getattr(eng_a.two, name)), 1e-12, 1e-14, f"2-D {name}")
+    assert_rel(_f64(eng_b.one.A), _f64(eng_a.one.A), 1e-12, 1e-14, "A")
+    assert_rel(_f64(eng_b.one.Q), _f64(eng_a.one.Q), 1e-12, 1e-12, "Q")
+
+
 def test_device_engine_matches_oracle(gpu):
     """Device engine vs the float64 CPU oracle running the same coupled order, 30 steps."""
     from oracle.oracle import Oracle2D, OracleCoupled, OracleNet, oracle_links
