@@ -69,6 +69,7 @@ def engine_throughput(wl, steps: int, warmup: int):
     eng.check_plausible(t_sim, "bench")
     cells = wl.batch * wl.cells
     return dict(cells=cells, wall_s=wall, event_ms=ms_ev, rate=cells * steps / wall, t_sim=t_sim,
+                shape=tuning.describe(eng.two.dtype, shape),
                 mass_err=float(max(abs(float(x)) for x in (err if hasattr(err, "__len__") else [err]))),
                 ms_per_step=1e3 * wall / steps, eng=eng)
 
@@ -129,14 +130,45 @@ def cpu_baseline(budget_s: float = 20.0):
                        f"({wall:.1f} s wall, OpenMP over members with {cores} threads)")
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "swe2d_step_f32_400x325_b32_traffic.md"
+
+
 def traffic_from_profile():
-    p = ROOT / "profiles" / "traffic_swe2d_f32.json"
-    if not p.exists():
+    """HBM bytes per 2-D step from the committed FETCH_SIZE/WRITE_SIZE summary (line 'traffic_bytes_per_step: N')."""
+    if not TRAFFIC_FILE.exists():
         return None
-    try:
-        return json.loads(p.read_text()).get("bytes_per_step")
-    except Exception:
-        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes_per_step:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def kernel_only(reps: int):
+    """The 2-D step of the headline workload alone, then a 256 MiB copy of known size: the run the
+    FETCH_SIZE / WRITE_SIZE counter passes profile (the copy calibrates the counters' units)."""
+    import torch
+    from his_amd import tuning
+    from his_amd import workloads as W
+    from his_amd.engine import build_from_workload
+    wl = W.precision_form()
+    eng, _ = build_from_workload(wl, device=torch.device("cuda"))
+    shape = tuning.tune(eng.two)
+    log(f"[kernel-only] shape {shape} ({tuning.describe(eng.two.dtype, shape)}), {reps} steps")
+    de = eng._device()          # advance the storm 2000 coupled steps so the step is timed on a wet state
+    de.t.fill_(0.0)
+    de.t_end.fill_(1e12)
+    de.t_limit.fill_(1e12)
+    de.refresh()
+    de.step(2000)
+    torch.cuda.synchronize()
+    us, gbs = kernel_roofline(eng, reps, ALG_BYTES_F32)
+    log(f"[kernel-only] 2-D step {us:.1f} us, {gbs:.0f} GB/s algorithmic")
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    b = torch.ones_like(a)
+    for _ in range(3):
+        a.copy_(b)
+    torch.cuda.synchronize()
+    log("[kernel-only] copy 256 MiB x 3 done")
 
 
 def main():
@@ -147,12 +179,17 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1000)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="run only the 2-D step (and a calibration copy) for counter profiling; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the storms, members and requests of this engine are independent and "
               "would shard across GPUs with no exchange; multi-GPU runs are not part of this build.",
               file=sys.stderr)
         sys.exit(2)
+    if args.kernel_only:
+        kernel_only(args.steps)
+        return
     import torch
     from his_amd import workloads as W
 
@@ -189,7 +226,8 @@ def main():
                    "latency_p50_p99": None},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(),
-                     "kernel": "fused 2-D step: k_stage<1> + k_stage<2> + finalize, 56 B per member-cell",
+                     "kernel": f"2-D step ({r32['shape']}): stage-1 + stage-2 kernels + finalize, "
+                               "56 B per member-cell",
                      "kernel_us": us},
         "cpu_baseline": cpu,
     }
