@@ -20,6 +20,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -72,6 +74,42 @@ def engine_throughput(wl, steps: int, warmup: int):
                 shape=tuning.describe(eng.two.dtype, shape),
                 mass_err=float(max(abs(float(x)) for x in (err if hasattr(err, "__len__") else [err]))),
                 ms_per_step=1e3 * wall / steps, eng=eng)
+
+
+def batching_throughput(steps: int, warmup: int):
+    """Three uneven grids (16 x 100x81 @ 80 m, 8 x 200x163 @ 40 m, 8 x 400x325 @ 20 m, float32
+    bulk) in one engine: one launch per 2-D stage over all of them, one dt per grid group."""
+    import torch
+    from his_amd.batching import build_batching_form, member_cell_updates
+    ge, _ = build_batching_form(device=torch.device("cuda"), tune=True)
+    de = ge.dev
+    de.t.fill_(0.0)
+    de.t_end.fill_(1e12)
+    de.t_limit.fill_(1e12)
+    de.refresh()
+    s0 = ge.steps()
+    de.step(warmup)
+    torch.cuda.synchronize()
+    s1 = ge.steps()
+    w0 = time.perf_counter()
+    de.step(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    s2 = ge.steps()
+    if any(b - a != warmup for a, b in zip(s0, s1)) or any(b - a != steps for a, b in zip(s1, s2)):
+        raise RuntimeError(f"grouped engine took {s1} / {s2} steps from {s0}")
+    ge._fold()
+    t_sim = de.t.cpu().numpy()
+    errs = []
+    for gi, e in enumerate(ge.engines):
+        m0 = ge.m0[gi]
+        t_g = float(t_sim[m0])
+        errs.append(float(np.max(np.abs(np.asarray(e.mb.check(e.total_volume(), t_g))))))
+        e.check_plausible(t_g, f"bench group {gi}")
+    cu = member_cell_updates(ge, [steps] * len(ge.engines))
+    ge.close()
+    return dict(rate=cu / wall, ms_per_step=1e3 * wall / steps, member_cells=ge.member_cells,
+                mass_err=max(errs), groups=[f"{e.two.B}x{e.two.nx}x{e.two.ny}" for e in ge.engines])
 
 
 def kernel_roofline(eng, reps: int, alg_bytes: int):
@@ -207,6 +245,10 @@ def main():
     us64, gbs64 = kernel_roofline(r64["eng"], 100, ALG_BYTES_F64)
     del r64["eng"]
     log(f"[bench]   {r64['rate'] / 1e9:.3f} G cell-updates/s, 2-D step {us64:.1f} us ({gbs64:.0f} GB/s)")
+    log("[bench] batching form: 16x100x81 + 8x200x163 + 8x400x325, one engine")
+    rb = batching_throughput(max(args.steps // 2, 1), max(args.warmup // 2, 1))
+    log(f"[bench]   {rb['rate'] / 1e9:.3f} G cell-updates/s, {rb['ms_per_step']:.4f} ms/step, "
+        f"mass err {rb['mass_err']:.2e}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (float64 oracle)")
@@ -223,6 +265,9 @@ def main():
                    "fp64_plain": {"value": r64["rate"], "ms_per_step": r64["ms_per_step"],
                                   "kernel_us": us64, "kernel_gbs_alg": gbs64, "mass_err": r64["mass_err"]},
                    "fp32_over_fp64_plain": r32["rate"] / r64["rate"],
+                   "batching": {"value": rb["rate"], "ms_per_step": rb["ms_per_step"],
+                                "member_cells": rb["member_cells"], "groups": rb["groups"],
+                                "mass_err": rb["mass_err"]},
                    "latency_p50_p99": None},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(),

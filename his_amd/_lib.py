@@ -92,7 +92,8 @@ SYMBOLS = {
     "his_engine_fold_max": (C.c_int, [VP, C.c_int32, VP, VP]),
     "his_engine_destroy": (C.c_int, [VP]),
     "his_set_tile_shape": (C.c_int, [C.c_int32, C.c_int32]),
-    "his_tile_shape": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "his_tile_shape": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]),
     "his_last_error": (C.c_char_p, []),
     "his_version": (C.c_char_p, []),
 }
@@ -155,6 +156,8 @@ def set_tile_shape(dtype: torch.dtype, shape: int) -> None:
 
 
 def tile_shape(dtype: torch.dtype, shape: int):
-    tx, ty = C.c_int32(), C.c_int32()
-    check(load().his_tile_shape(dtype_code(dtype), int(shape), C.byref(tx), C.byref(ty)), "tile_shape")
-    return tx.value, ty.value
+    """(tx, ty, variant) of a compiled 2-D kernel shape; variant 0 tile, 1 LDS march, 2 register march."""
+    tx, ty, var = C.c_int32(), C.c_int32(), C.c_int32()
+    check(load().his_tile_shape(dtype_code(dtype), int(shape), C.byref(tx), C.byref(ty), C.byref(var)),
+          "tile_shape")
+    return tx.value, ty.value, var.value

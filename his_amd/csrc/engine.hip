@@ -540,28 +540,22 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
     int rc;
     hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(64), 0, s, e->D);
     HIS_HIP(hipGetLastError());
+    // every group's weir exchange: one launch per phase
+    ExGroups X{};
+    Swe1DGroups W{};
     for (int gi = 0; gi < e->d.ngroups; ++gi) {
         if (!e->has_net[gi]) continue;
         const his_group& G = e->d.group[gi];
-        ExchangeArgs a{};
+        const int m0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
+        ExchangeArgs& a = X.a[X.ngroups++];
         a.L = e->links[gi]; a.N = e->net[gi]; a.dtype2d = e->dtype; a.plane = (int64_t)G.grid.ny * G.grid.nx;
         a.h = G.state.h; a.hu = G.state.hu; a.hv = G.state.hv; a.A = G.A; a.dt = e->d.ctl.dt; a.V = nullptr;
         a.acc = nullptr; a.acc_slot_in = 8; a.acc_stride = e->Btot; a.active = e->active; a.ws = e->wsx[gi];
-        a.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
-        if ((rc = launch_exchange(a, s))) return rc;
-    }
-    // 1-D branch beside the 2-D stages
-    bool any_net = false;
-    for (int gi = 0; gi < e->d.ngroups; ++gi) any_net |= e->has_net[gi] != 0;
-    hipStream_t s1 = any_net && side ? side : s;
-    if (any_net && side) {
-        HIS_HIP(hipEventRecord(e->ev_fork, s));
-        HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
-    }
-    for (int gi = 0; gi < e->d.ngroups; ++gi) {
-        if (!e->has_net[gi]) continue;
-        const his_group& G = e->d.group[gi];
-        Net1DDrive d{};
+        a.member0 = m0;
+        const int k = W.ngroups++;
+        W.N[k] = e->net[gi]; W.A[k] = G.A; W.Q[k] = G.Q; W.ws[k] = e->ws1d[gi];
+        W.blk0[k + 1] = W.blk0[k] + G.grid.B;
+        Net1DDrive& d = W.d[k];
         d.dt = e->d.ctl.dt; d.engine = 1; d.n_sub = e->d.ctl.n_sub; d.t = e->t_step;
         d.t_out = e->d.ctl.t; d.steps = e->d.ctl.steps;
         d.ex_vf = exchange_vf(e->wsx[gi], e->links[gi], e->net[gi], 0);
@@ -569,9 +563,17 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
         d.ex_stride = 3 * e->links[gi].nl + 2 * e->links[gi].nu + e->net[gi].nc;
         d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
-        d.member0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
-        if ((rc = launch_swe1d(e->net[gi], G.A, G.Q, d, e->ws1d[gi], G.grid.B, s1))) return rc;
+        d.member0 = m0;
     }
+    if ((rc = launch_exchange_groups(X, s))) return rc;
+    // every group's 1-D network in one launch, on a side stream beside the 2-D stages
+    const bool any_net = W.ngroups > 0;
+    hipStream_t s1 = any_net && side ? side : s;
+    if (any_net && side) {
+        HIS_HIP(hipEventRecord(e->ev_fork, s));
+        HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
+    }
+    if ((rc = launch_swe1d_groups(W, s1))) return rc;
     rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
     if (rc) return rc;
     if ((rc = launch_finalize(e->F, s))) return rc;
@@ -840,11 +842,12 @@ extern "C" int his_set_tile_shape(int32_t dtype, int32_t shape) {
     return 0;
 }
 
-extern "C" int his_tile_shape(int32_t dtype, int32_t shape, int32_t* tx, int32_t* ty) {
+extern "C" int his_tile_shape(int32_t dtype, int32_t shape, int32_t* tx, int32_t* ty, int32_t* variant) {
     if ((dtype != HIS_F64 && dtype != HIS_F32) || shape < 0 || shape >= kNumShapes || !tx || !ty) return HIS_E_ARG;
     const TileShape& t = dtype == HIS_F64 ? kShapesF64[shape] : kShapesF32[shape];
     *tx = t.tx;
     *ty = t.ty;
+    if (variant) *variant = t.march;
     return 0;
 }
 

@@ -102,11 +102,10 @@ __device__ void rate1d(const his_net1d& N, int b, const double* A, const double*
     }
 }
 
+// All substeps of one coupled step for member b of one network (one workgroup).
 template <int NT>
-__global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restrict__ Ag, double* __restrict__ Qg,
-                                               const Net1DDrive d, double* __restrict__ wsg) {
-    __shared__ double red[NT / 64][8];
-    const int b = blockIdx.x;
+__device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double* __restrict__ Qg,
+                             const Net1DDrive& d, double* __restrict__ wsg, const int b, double (*red)[8]) {
     const int m = d.member0 + b;
     if (d.active && !d.active[m]) return;
     const int nc = N.nc, nf = N.nf;
@@ -287,6 +286,22 @@ __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restr
 }
 
 template <int NT>
+__global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restrict__ Ag, double* __restrict__ Qg,
+                                               const Net1DDrive d, double* __restrict__ wsg) {
+    __shared__ double red[NT / 64][8];
+    swe1d_member<NT>(N, Ag, Qg, d, wsg, blockIdx.x, red);
+}
+
+// Every group's network in one launch: workgroups blk0[g] .. blk0[g+1]-1 are group g's members.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_swe1d_groups(const Swe1DGroups G) {
+    __shared__ double red[NT / 64][8];
+    int gi = 0;
+    while (gi + 1 < G.ngroups && (int)blockIdx.x >= G.blk0[gi + 1]) ++gi;
+    swe1d_member<NT>(G.N[gi], G.A[gi], G.Q[gi], G.d[gi], G.ws[gi], (int)blockIdx.x - G.blk0[gi], red);
+}
+
+template <int NT>
 __global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const double* Ag, const double* Qg,
                                                     double* rate, int32_t* wet, int member0) {
     __shared__ double red[NT / 64][8];
@@ -299,6 +314,13 @@ size_t swe1d_ws_doubles(const his_net1d& n) { return (size_t)n.B * (3 * (size_t)
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
                  hipStream_t s) {
     hipLaunchKernelGGL(k_swe1d<1024>, dim3(nblocks), dim3(1024), 0, s, n, A, Q, d, ws);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s) {
+    if (g.ngroups < 1) return 0;
+    hipLaunchKernelGGL(k_swe1d_groups<1024>, dim3(g.blk0[g.ngroups]), dim3(1024), 0, s, g);
     HIS_HIP(hipGetLastError());
     return 0;
 }
@@ -334,13 +356,23 @@ __device__ __forceinline__ ExWs ex_ws(const ExchangeArgs& a, int b) {
     return ExWs{w, w + nl, w + 2 * nl, w + 3 * nl, w + 3 * nl + nu, w + 3 * nl + 2 * nu};
 }
 
+// thread -> (group, item within the group) for a grid that spans every group's items
+__device__ __forceinline__ int ex_group(const ExGroups& G, const int64_t* it0, int64_t it) {
+    int gi = 0;
+    while (gi + 1 < G.ngroups && it >= it0[gi + 1]) ++gi;
+    return gi;
+}
+
 template <typename R2>
-__global__ void __launch_bounds__(256) k_ex_links(const ExchangeArgs a) {
+__global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
+    int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= G.it_links[G.ngroups]) return;
+    const int gi = ex_group(G, G.it_links, it);
+    const ExchangeArgs& a = G.a[gi];
+    it -= G.it_links[gi];
     const his_links& L = a.L;
     const his_net1d& N = a.N;
     const int per = L.nl + L.nu;
-    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (it >= (int64_t)N.B * per) return;
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     if (a.active && !a.active[m]) return;
@@ -370,12 +402,15 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExchangeArgs a) {
     w.V[l] = tmin(v, L.relax * v_eq);
 }
 
-__global__ void __launch_bounds__(256) k_ex_limits(const ExchangeArgs a) {
+__global__ void __launch_bounds__(256) k_ex_limits(const ExGroups G) {
+    int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= G.it_cells[G.ngroups]) return;
+    const int gi = ex_group(G, G.it_cells, it);
+    const ExchangeArgs& a = G.a[gi];
+    it -= G.it_cells[gi];
     const his_links& L = a.L;
     const his_net1d& N = a.N;
     const int per = L.nu + N.nc;
-    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (it >= (int64_t)N.B * per) return;
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     if (a.active && !a.active[m]) return;
@@ -408,12 +443,15 @@ __device__ __forceinline__ double ex_final(const ExWs& w, const his_links& L, in
 }
 
 template <typename R2>
-__global__ void __launch_bounds__(256) k_ex_apply(const ExchangeArgs a) {
+__global__ void __launch_bounds__(256) k_ex_apply(const ExGroups G) {
+    int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (it >= G.it_cells[G.ngroups]) return;
+    const int gi = ex_group(G, G.it_cells, it);
+    const ExchangeArgs& a = G.a[gi];
+    it -= G.it_cells[gi];
     const his_links& L = a.L;
     const his_net1d& N = a.N;
     const int per = L.nu + N.nc;
-    const int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (it >= (int64_t)N.B * per) return;
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     if (a.active && !a.active[m]) return;
@@ -470,23 +508,39 @@ __global__ void __launch_bounds__(NT) k_ex_ledger(const ExchangeArgs a) {
     }
 }
 
-int launch_exchange(const ExchangeArgs& a, hipStream_t s) {
-    const int B = a.N.B;
+int launch_exchange_groups(ExGroups& G, hipStream_t s) {
+    if (G.ngroups < 1) return 0;
+    G.it_links[0] = G.it_cells[0] = 0;
+    for (int gi = 0; gi < G.ngroups; ++gi) {
+        const ExchangeArgs& a = G.a[gi];
+        if (a.dtype2d != G.a[0].dtype2d) return HIS_E_ARG;
+        G.it_links[gi + 1] = G.it_links[gi] + (int64_t)a.N.B * (a.L.nl + a.L.nu);
+        G.it_cells[gi + 1] = G.it_cells[gi] + (int64_t)a.N.B * (a.L.nu + a.N.nc);
+    }
     auto blocks = [](int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); };
-    const dim3 g1 = blocks((int64_t)B * (a.L.nl + a.L.nu)), g2 = blocks((int64_t)B * (a.L.nu + a.N.nc));
-    if (a.dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, a);
+    const dim3 g1 = blocks(G.it_links[G.ngroups]), g2 = blocks(G.it_cells[G.ngroups]);
+    if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, G);
+    else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ex_limits, g2, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_ex_limits, g2, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());
-    if (a.dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, a);
+    if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, G);
+    else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());
-    if (a.acc) {
-        hipLaunchKernelGGL(k_ex_ledger<256>, dim3(B), dim3(256), 0, s, a);
+    for (int gi = 0; gi < G.ngroups; ++gi) {
+        const ExchangeArgs& a = G.a[gi];
+        if (!a.acc) continue;
+        hipLaunchKernelGGL(k_ex_ledger<256>, dim3(a.N.B), dim3(256), 0, s, a);
         HIS_HIP(hipGetLastError());
     }
     return 0;
+}
+
+int launch_exchange(const ExchangeArgs& a, hipStream_t s) {
+    ExGroups G{};
+    G.ngroups = 1;
+    G.a[0] = a;
+    return launch_exchange_groups(G, s);
 }
 
 // ------------------------------------------------------------------ plausibility

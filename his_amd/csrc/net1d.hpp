@@ -52,6 +52,24 @@ struct ExchangeArgs {
 };
 int launch_exchange(const ExchangeArgs& a, hipStream_t s);
 
+// Several groups' exchanges in one launch per phase (every group shares the 2-D dtype).
+struct ExGroups {
+    int ngroups;
+    int64_t it_links[HIS_MAX_GROUPS + 1], it_cells[HIS_MAX_GROUPS + 1];   // filled by the launcher
+    ExchangeArgs a[HIS_MAX_GROUPS];
+};
+int launch_exchange_groups(ExGroups& g, hipStream_t s);
+
+// Several groups' 1-D networks in one launch: workgroups blk0[g] .. blk0[g+1]-1 run group g.
+struct Swe1DGroups {
+    int ngroups;
+    int blk0[HIS_MAX_GROUPS + 1];
+    his_net1d N[HIS_MAX_GROUPS];
+    double *A[HIS_MAX_GROUPS], *Q[HIS_MAX_GROUPS], *ws[HIS_MAX_GROUPS];
+    Net1DDrive d[HIS_MAX_GROUPS];
+};
+int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s);
+
 size_t swe1d_ws_doubles(const his_net1d& n);
 // signed link volumes of the last exchange for member b (float64, nl values) inside its workspace
 inline const double* exchange_vf(double* ws, const his_links& l, const his_net1d& n, int b) {

@@ -83,5 +83,5 @@ def tune(two, reps: int = 20, force: bool = False) -> int:
 
 
 def describe(dtype, shape: int) -> str:
-    tx, ty = L.tile_shape(dtype, shape)
-    return f"march {tx}x{ty}" if tx == 58 else f"tile {tx}x{ty}"
+    tx, ty, var = L.tile_shape(dtype, shape)
+    return f"{('tile', 'march', 'wave')[var]} {tx}x{ty}"

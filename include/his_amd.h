@@ -232,9 +232,11 @@ int his_engine_destroy(his_engine *e);
 /* ------------------------------------------------------------------ misc */
 
 /* Tile shape of the 2-D kernels, per dtype: index into the compiled shapes (0 = default).
- * The autotuner (his_amd/tuning.py) times each and stores the fastest per grid. */
+ * The autotuner (his_amd/tuning.py) times each and stores the fastest per grid.
+ * his_tile_shape reports a shape: tile or strip width tx, rows ty, and the kernel variant
+ * (0 LDS-tiled, 1 LDS-ring march, 2 register march); variant may be NULL. */
 int his_set_tile_shape(int32_t dtype, int32_t shape);
-int his_tile_shape(int32_t dtype, int32_t shape, int32_t *tx, int32_t *ty);
+int his_tile_shape(int32_t dtype, int32_t shape, int32_t *tx, int32_t *ty, int32_t *variant);
 
 const char *his_last_error(void);
 const char *his_version(void);
