@@ -112,6 +112,22 @@ def batching_throughput(steps: int, warmup: int):
                 mass_err=max(errs), groups=[f"{e.two.B}x{e.two.nx}x{e.two.ny}" for e in ge.engines])
 
 
+def serving_latency(n: int):
+    """The first n requests of the seeded Poisson(4/s) stream (grid size uniform over 100x81 @ 80 m,
+    200x163 @ 40 m, 400x325 @ 20 m; storm U[0.5, 6] h), served in real time; latency = completion -
+    arrival."""
+    from his_amd import serving as S
+    reqs = S.requests_for(n)
+    server = S.Server()
+    w0 = time.perf_counter()
+    res = server.serve(reqs, log_every_s=0)
+    wall = time.perf_counter() - w0
+    server.close()
+    out = S.summarize(res, wall)
+    out["sample"] = f"first {n} of the 2000-request stream (seed 5)"
+    return out
+
+
 def kernel_roofline(eng, reps: int, alg_bytes: int):
     """Time the fused 2-D step (two stage kernels + finalize) with HIP events on its stream."""
     import torch
@@ -217,6 +233,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=1000)
+    ap.add_argument("--serve", type=int, default=120, help="requests of the serving sample (0 = skip)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the 2-D step (and a calibration copy) for counter profiling; prints no JSON")
     args = ap.parse_args()
@@ -249,6 +266,12 @@ def main():
     rb = batching_throughput(max(args.steps // 2, 1), max(args.warmup // 2, 1))
     log(f"[bench]   {rb['rate'] / 1e9:.3f} G cell-updates/s, {rb['ms_per_step']:.4f} ms/step, "
         f"mass err {rb['mass_err']:.2e}")
+    srv = None
+    if args.serve:
+        log(f"[bench] serving form: {args.serve} Poisson(4/s) requests over 3 grid sizes, continuous batching")
+        srv = serving_latency(args.serve)
+        log(f"[bench]   p50 {srv['latency_p50_s']:.2f} s, p99 {srv['latency_p99_s']:.2f} s, "
+            f"{srv['ok']}/{srv['requests']} ok, worst mass err {srv['worst_mass_err']:.1e}")
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline (float64 oracle)")
@@ -268,7 +291,7 @@ def main():
                    "batching": {"value": rb["rate"], "ms_per_step": rb["ms_per_step"],
                                 "member_cells": rb["member_cells"], "groups": rb["groups"],
                                 "mass_err": rb["mass_err"]},
-                   "latency_p50_p99": None},
+                   "serving": srv},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(),
                      "kernel": f"2-D step ({r32['shape']}): stage-1 + stage-2 kernels + finalize, "

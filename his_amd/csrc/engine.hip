@@ -42,6 +42,7 @@ __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
     __shared__ double red[NT / 64][24];
     const int m = blockIdx.x;
     if (F.active && !F.active[m]) return;
+    if (F.only && !F.only[m]) return;
     int gi = 0;
     while (gi + 1 < F.ngroups && m >= F.member0[gi + 1]) ++gi;
     const int b = m - F.member0[gi];
@@ -361,7 +362,7 @@ extern "C" int his_swe1d_max_rate(const his_net1d* n, const double* A, const dou
     int rc = check_net(n);
     if (rc) return rc;
     if (!A || !Q || !rate || !any_wet) { set_error("his_swe1d_max_rate: NULL pointer"); return HIS_E_ARG; }
-    return launch_swe1d_rate(*n, A, Q, rate, any_wet, 0, (hipStream_t)stream);
+    return launch_swe1d_rate(*n, A, Q, rate, any_wet, 0, (hipStream_t)stream, nullptr);
 }
 
 static int check_links(const his_links* l, const his_net1d* n) {
@@ -749,9 +750,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     return 0;
 }
 
-extern "C" int his_engine_refresh(his_engine* e, void* stream) {
-    if (!e) { set_error("NULL engine"); return HIS_E_ARG; }
-    hipStream_t s = (hipStream_t)stream;
+static int engine_refresh(his_engine* e, const int32_t* mask, hipStream_t s) {
     int rc;
     if (e->dtype == HIS_F64) rc = launch_state_reduce(e->L64, e->shape, s);
     else rc = launch_state_reduce(e->L32, e->shape, s);
@@ -760,14 +759,25 @@ extern "C" int his_engine_refresh(his_engine* e, void* stream) {
     F.stage_only = 1;
     F.active = nullptr;
     F.acc = nullptr;
+    F.only = mask;
     if ((rc = launch_finalize(F, s))) return rc;
     for (int gi = 0; gi < e->d.ngroups; ++gi) {
         if (!e->has_net[gi]) continue;
         const his_group& G = e->d.group[gi];
         const int m0 = e->dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0;
-        if ((rc = launch_swe1d_rate(e->net[gi], G.A, G.Q, e->rate1d, e->wet1d, m0, s))) return rc;
+        if ((rc = launch_swe1d_rate(e->net[gi], G.A, G.Q, e->rate1d, e->wet1d, m0, s, mask))) return rc;
     }
     return 0;
+}
+
+extern "C" int his_engine_refresh(his_engine* e, void* stream) {
+    if (!e) { set_error("NULL engine"); return HIS_E_ARG; }
+    return engine_refresh(e, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int his_engine_refresh_members(his_engine* e, const int32_t* mask, void* stream) {
+    if (!e || !mask) { set_error("his_engine_refresh_members: NULL argument"); return HIS_E_ARG; }
+    return engine_refresh(e, mask, (hipStream_t)stream);
 }
 
 static int engine_build_graphs(his_engine* e) {

@@ -303,9 +303,10 @@ __global__ void __launch_bounds__(NT) k_swe1d_groups(const Swe1DGroups G) {
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const double* Ag, const double* Qg,
-                                                    double* rate, int32_t* wet, int member0) {
+                                                    double* rate, int32_t* wet, int member0, const int32_t* only) {
     __shared__ double red[NT / 64][8];
     const int b = blockIdx.x;
+    if (only && !only[member0 + b]) return;
     rate1d<NT>(N, b, Ag + (int64_t)b * N.nc, Qg + (int64_t)b * N.nc, rate, wet, member0 + b, red);
 }
 
@@ -326,8 +327,8 @@ int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s) {
 }
 
 int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
-                      int member0, hipStream_t s) {
-    hipLaunchKernelGGL(k_swe1d_rate<1024>, dim3(n.B), dim3(1024), 0, s, n, A, Q, rate, wet, member0);
+                      int member0, hipStream_t s, const int32_t* only) {
+    hipLaunchKernelGGL(k_swe1d_rate<1024>, dim3(n.B), dim3(1024), 0, s, n, A, Q, rate, wet, member0, only);
     HIS_HIP(hipGetLastError());
     return 0;
 }

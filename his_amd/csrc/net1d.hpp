@@ -32,7 +32,7 @@ struct Net1DDrive {
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
                  hipStream_t s);
 int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
-                      int member0, hipStream_t s);
+                      int member0, hipStream_t s, const int32_t* only);   // only: NULL or (Btot) member mask
 
 struct ExchangeArgs {
     his_links L;

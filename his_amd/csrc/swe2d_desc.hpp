@@ -78,6 +78,7 @@ struct FinalizeArgs {
     double dx[HIS_MAX_GROUPS], dy[HIS_MAX_GROUPS], area[HIS_MAX_GROUPS], cells[HIS_MAX_GROUPS];
     int rain_f32[HIS_MAX_GROUPS];        // rain * dt formed in float32 (the working dtype) before widening
     int stage_only;                      // 1 = only rate/volume (after a state reduction)
+    const int32_t* only;                 // stage_only: members with only[m] == 0 are left untouched
     // engine: members of groups without a 1-D network advance here, t_out = t_step + dt, steps += 1
     double* t_out;
     const double* t_step;

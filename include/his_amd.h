@@ -223,6 +223,12 @@ int his_engine_step(his_engine *e, int32_t n_steps, void *stream);
 /* Recompute the CFL rates after the host has changed any state (set_state, exchange by hand). */
 int his_engine_refresh(his_engine *e, void *stream);
 
+/* The same for the members with mask[m] != 0 only (mask: device (Btot) int32); the others keep the
+ * rates their last step left, so loading a new storm into one slot leaves every running storm's
+ * step sequence exactly as it would be alone.  No counterpart in hydrointel, which runs one batch
+ * at a time (hydrointel/api.py:274-324); this serves continuous batching. */
+int his_engine_refresh_members(his_engine *e, const int32_t *mask, void *stream);
+
 /* Fold each member's current depth into a running maximum (device, grid dtype), for the
  * depth_max of FloodResult taken over output times (hydrointel/api.py:239). */
 int his_engine_fold_max(his_engine *e, int32_t group, void *hmax, void *stream);

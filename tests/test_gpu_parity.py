@@ -394,3 +394,34 @@ def test_grouped_engine_equals_each_group_alone(gpu):
         sa, sb = a.mb.sources(), b.mb.sources()
         for k in sa:
             assert_rel(np.asarray(sb[k]), np.asarray(sa[k]), 1e-13, 1e-12, f"group {gi} ledger {k}")
+
+
+@pytest.mark.parametrize("precision", ["fp32bulk", "fp64"])
+def test_served_request_equals_the_request_alone(gpu, precision):
+    """A request served among others -- sharing launches with busy slots of two grid sizes, loaded
+    into a slot another request used before -- gives exactly its standalone result."""
+    from his_amd import serving as S
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    grids = (S.GridSpec(100, 81, 80.0, "chain"), S.GridSpec(200, 163, 40.0, "net60"))
+    ms = W.draw_members(4, 77)
+    reqs = [dict(id=0, arrival_s=0.0, size=0, storm_s=1800.0, member=ms[0]),
+            dict(id=1, arrival_s=0.0, size=1, storm_s=1200.0, member=ms[1]),
+            dict(id=2, arrival_s=0.0, size=0, storm_s=960.0, member=ms[2]),
+            dict(id=3, arrival_s=0.0, size=1, storm_s=1800.0, member=ms[3])]
+    srv = S.Server(slots=(1, 2), precision=precision, device=gpu, grids=grids, chunk=16, keep_fields=True,
+                   tune=False)
+    res = {r.id: r for r in srv.serve(reqs, realtime=False, log_every_s=0)}
+    srv.close()
+    assert sorted(res) == [0, 1, 2, 3]
+    cfg = SolverConfig().replace(precision=precision)
+    for req in reqs:
+        r = res[req["id"]]
+        assert r.status == "ok", (req["id"], r.status)
+        alone = S.run_alone(grids[req["size"]], req, cfg, gpu)
+        assert r.steps == alone["steps"], (req["id"], r.steps, alone["steps"])
+        assert np.array_equal(r.fields["h"], alone["h"]), f"request {req['id']} h"
+        assert np.array_equal(r.fields["A"], alone["A"]), f"request {req['id']} A"
+        assert np.array_equal(r.fields["depth_max"].astype(np.float32), alone["depth_max"]), \
+            f"request {req['id']} depth max"
+        assert abs(r.mass_err - alone["mass_err"]) <= 1e-12 + 1e-9 * alone["mass_err"]

@@ -94,3 +94,21 @@ def test_solvers_refuse_to_run_without_a_gpu():
     from his_amd.solver.swe2d import SWE2D
     with pytest.raises(RuntimeError, match="ROCm GPU"):
         SWE2D(np.zeros((4, 4)), 0.03, 10.0, SolverConfig(), all_edges(), "cpu")
+
+
+def test_serving_tables_and_output_times():
+    """Serving storms sit on the knot grid (peak on a knot), outputs every 15 min plus the end."""
+    from his_amd import serving as S
+    assert S.storm_length(1801.0) == 1800.0 and S.storm_length(10.0) == 2 * S.STEP_S
+    assert S.storm_length(9 * 3600.0) == S.MAX_STORM_S
+    v = S.triangle_knots(1800.0, 5.0, 1.0)
+    t = np.arange(S.K_KNOTS) * S.STEP_S
+    assert v.shape == (S.K_KNOTS,) and v[0] == 1.0 and v[15] == 5.0 and v[30] == 1.0 and (v[31:] == 1.0).all()
+    # linear between knots: the closed form of the triangle at every knot
+    ref = 1.0 + 4.0 * np.maximum(0.0, 1.0 - np.abs(t - 900.0) / 900.0)
+    assert np.allclose(v, ref, rtol=0, atol=1e-15)
+    assert S.output_times(1800.0) == [0.0, 900.0, 1800.0]
+    assert S.output_times(1000.0) == [0.0, 900.0, 1000.0]
+    fo = S.request_forcing(dict(storm_s=1800.0, member=S.W.Member(rain_peak_mmh=72.0, inflow_peak=200.0)))
+    assert abs(float(fo.rain(900.0)[0]) - 72.0 / 3.6e6) < 1e-18 and float(fo.rain(1800.0)[0]) == 0.0
+    assert abs(float(fo.inflow(450.0)[0]) - 105.0) < 1e-12 and fo.t_end == 1800.0
