@@ -156,6 +156,15 @@ def set_tile_shape(dtype: torch.dtype, shape: int) -> None:
     check(load().his_set_tile_shape(dtype_code(dtype), int(shape)), "set_tile_shape")
 
 
+def num_tile_shapes() -> int:
+    """Number of compiled 2-D kernel shapes (the same for both precisions)."""
+    tx, ty = C.c_int32(), C.c_int32()
+    n = 0
+    while load().his_tile_shape(HIS_F32, n, C.byref(tx), C.byref(ty), None) == 0:
+        n += 1
+    return n
+
+
 def tile_shape(dtype: torch.dtype, shape: int):
     """(tx, ty, variant) of a compiled 2-D kernel shape; variant 0 tile, 1 LDS march, 2 register march."""
     tx, ty, var = C.c_int32(), C.c_int32(), C.c_int32()

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "../../include/his_amd.h"
 
 #define HIS_SQRT2 1.4142135623730951

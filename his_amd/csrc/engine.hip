@@ -502,13 +502,24 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
         a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
         a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
         a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
+    }
+    if (!act) return;
+    // the forcing of the step, one (member, slot) per thread: rain at t, then W, E, S, N at t and t + dt
+    const int nm = G.m1 - G.m0;
+    for (int k = threadIdx.x; k < nm * 9; k += blockDim.x) {
+        const int mi = k / 9, slot = k - mi * 9;
+        const int m = G.m0 + mi;
         const int b = m - a.member0[gi];
-        a.pm[m] = series_eval(a.rain[gi], b, t);
-        for (int e = 0; e < 4; ++e) {
+        const double t = a.ctl.t[m];
+        double v = 0.0;
+        if (slot == 0) {
+            v = series_eval(a.rain[gi], b, t);
+        } else {
+            const int e = (slot - 1) & 3;
             const bool valued = a.bc[gi][e] == HIS_BC_STAGE || a.bc[gi][e] == HIS_BC_DISCHARGE;
-            a.pm[(1 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t) : 0.0;
-            a.pm[(5 + e) * a.Btot + m] = valued ? series_eval(a.edge[gi][e], b, t + dt) : 0.0;
+            if (valued) v = series_eval(a.edge[gi][e], b, slot <= 4 ? t : t + dt);
         }
+        a.pm[slot * a.Btot + m] = v;
     }
 }
 
@@ -539,7 +550,7 @@ struct his_engine {
 
 static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
     int rc;
-    hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(64), 0, s, e->D);
+    hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(e->n_dtg == 1 ? 256 : 64), 0, s, e->D);
     HIS_HIP(hipGetLastError());
     // every group's weir exchange: one launch per phase
     ExGroups X{};

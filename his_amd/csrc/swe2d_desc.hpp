@@ -46,11 +46,18 @@ struct Launch2D {
 //   march = 1: one wave marches up a strip of TX = 58 columns for TY rows with LDS row rings (k_march)
 //   march = 2: the same walk with each lane's column held in registers, x-neighbours by shuffle (k_wave)
 struct TileShape { int tx, ty, nt, march; };
+// Chunk heights between 8 and 16 trade the march's warm-up rows (H + 6 row steps for H rows)
+// against waves per SIMD; on 400 x 325 grids H = 13 divides the rows exactly.
 constexpr TileShape kShapesF32[] = {{58, 16, 64, 2}, {58, 32, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
-                                    {64, 8, 256, 0}, {64, 16, 256, 0}};
+                                    {64, 8, 256, 0}, {64, 16, 256, 0}, {58, 10, 64, 2}, {58, 12, 64, 2},
+                                    {58, 13, 64, 2}};
+// float64 runs mostly at B = 1, where small tiles are what fills the chip
 constexpr TileShape kShapesF64[] = {{58, 16, 64, 2}, {58, 32, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
-                                    {32, 8, 256, 0}, {64, 4, 256, 0}};
-constexpr int kNumShapes = 6;
+                                    {32, 8, 256, 0}, {64, 4, 256, 0}, {16, 8, 128, 0}, {32, 4, 128, 0},
+                                    {58, 4, 64, 2}};
+constexpr int kNumShapes = 9;
+static_assert(sizeof(kShapesF32) / sizeof(TileShape) == kNumShapes, "float32 shape table");
+static_assert(sizeof(kShapesF64) / sizeof(TileShape) == kNumShapes, "float64 shape table");
 
 // host launchers, defined once per precision (swe2d_f32.hip, swe2d_f64.hip)
 int launch_step(const Launch2D<float>& L, int shape, hipStream_t s);

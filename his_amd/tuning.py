@@ -17,7 +17,6 @@ import torch
 
 from . import _lib as L
 
-N_SHAPES = 6
 
 
 def _path() -> Path:
@@ -52,7 +51,7 @@ def time_shapes(two, reps: int = 20) -> dict:
     pm = two._pack([0.0] * 9)
     dt_t = two._pack([0.05, 0.0])
     out = {}
-    for shape in range(N_SHAPES):
+    for shape in range(L.num_tile_shapes()):
         L.set_tile_shape(two.dtype, shape)
         for _ in range(3):
             two._run(*st, dt_t, pm)

@@ -331,7 +331,7 @@ def test_fp32_bulk_step_tracks_fp64(golden, gpu):
     assert np.all(np.abs(vols - d["vols"][0]) <= 1e-4 * np.abs(d["vols"][0]) + 1e-3)
 
 
-@pytest.mark.parametrize("shape", range(6))
+@pytest.mark.parametrize("shape", range(9))
 def test_every_kernel_shape_matches_reference(golden, gpu, shape):
     """Tiled and marching kernel variants (the tuner's choices) all reproduce the reference step."""
     from his_amd import _lib
@@ -425,3 +425,36 @@ def test_served_request_equals_the_request_alone(gpu, precision):
         assert np.array_equal(r.fields["depth_max"].astype(np.float32), alone["depth_max"]), \
             f"request {req['id']} depth max"
         assert abs(r.mass_err - alone["mass_err"]) <= 1e-12 + 1e-9 * alone["mass_err"]
+
+
+def test_simulate_and_simulate_batch(gpu):
+    """FloodResult assembly: depth_max over the output-time snapshots, per-member mass balance,
+    a batch of one routed to simulate, and the zero disbenefit of a design against itself."""
+    from his_amd import api
+    from his_amd import workloads as W
+    from his_amd.engine import Forcing
+    wl = W.plain_cpu()
+    ter, net = wl.terrain, wl.network
+    wl.members = [wl.members[0], W.Member(rain_peak_mmh=60.0, tide_phase=0.7, inflow_peak=150.0)]
+    rain, tide, inflow = wl.forcing(batched=True)
+    fo2 = Forcing(rain, inflow, tide, wl.storm.t_end)
+    from his_amd.forcing import select
+    fo1 = Forcing(select(rain, [0]), select(inflow, [0]), select(tide, [0]), wl.storm.t_end)
+    site = api.baseline_site(ter, net)
+    r = api.simulate(site, fo1, ter, net, device=gpu)
+    nt = len(fo1.times(900.0))
+    assert r.depth_series.shape == (nt, ter.ny, ter.nx) and r.times.shape == (nt,)
+    assert torch.equal(r.depth_max, r.depth_series.amax(0))
+    assert r.mass_balance_error < 1e-3 and r.depth_max.max() > 0.1
+    assert r.channel_eta.shape[1] == int((r.extras["topology"].kind <= 1).sum())
+    (r1,) = api.simulate_batch([site], fo1, ter, net, device=gpu)
+    assert torch.equal(r1.depth_max, r.depth_max)
+    wetter = site.replace(storage_depth=site.storage_depth * 0.0)
+    rb = api.simulate_batch([site, wetter], fo2, ter, net, device=gpu)
+    assert len(rb) == 2 and all(x.mass_balance_error < 1e-3 for x in rb)
+    # member 0 of the batch shares dt with member 1, so it is close to, not equal to, the run alone
+    assert float((rb[0].depth_max - r.depth_max).abs().max()) < 0.05
+    d = api.disbenefit(r.depth_max, r.depth_max, ter, net)
+    assert d == {"max_depth_increase_m": 0.0, "area_worsened_ha": 0.0}
+    with pytest.raises(ValueError):
+        api.simulate(site.replace(manning=site.manning * 0.0), fo1, ter, net, device=gpu)

@@ -112,3 +112,14 @@ def test_serving_tables_and_output_times():
     fo = S.request_forcing(dict(storm_s=1800.0, member=S.W.Member(rain_peak_mmh=72.0, inflow_peak=200.0)))
     assert abs(float(fo.rain(900.0)[0]) - 72.0 / 3.6e6) < 1e-18 and float(fo.rain(1800.0)[0]) == 0.0
     assert abs(float(fo.inflow(450.0)[0]) - 105.0) < 1e-12 and fo.t_end == 1800.0
+
+
+def test_shape_table_matches_the_shape_tests():
+    """Every compiled 2-D kernel shape is exercised by the GPU parity test over shapes."""
+    import torch
+    from his_amd import _lib as L
+    n = L.num_tile_shapes()
+    src = open(__file__.replace("test_host.py", "test_gpu_parity.py")).read()
+    m = re.search(r'parametrize\("shape", range\((\d+)\)\)', src)
+    assert m and int(m.group(1)) == n
+    assert [L.tile_shape(torch.float32, k)[2] for k in range(n)].count(2) >= 3
