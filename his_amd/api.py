@@ -124,16 +124,15 @@ def _result(eng, snaps, err, b, batched: bool, prov: dict, extras: dict) -> Floo
     st = lambda k: torch.as_tensor(np.stack([pick(getattr(s, k)) for s in snaps]))
     h, u, v = st("h"), st("u"), st("v")
     src = eng.mb.sources()
-    s_b = {k: float(pick(np.asarray(v_))) for k, v_ in src.items()}
+    s_b = {k: float(np.asarray(v_, dtype=np.float64).reshape(-1)[b]) for k, v_ in src.items()}
     sv = eng.two.storage_volume().detach().cpu().numpy()
     return FloodResult(
         depth_max=h.amax(0), depth_series=h, u=u, v=v,
         speed_max=torch.sqrt(u.double() ** 2 + v.double() ** 2).amax(0).float(),
         channel_eta=st("eta1")[:, interior], channel_q=st("q1")[:, interior],
         times=torch.as_tensor(np.array([x.t for x in snaps])),
-        infiltrated_volume_m3=s_b["infiltration"], stored_volume_m3=float(pick(np.atleast_1d(sv)) if batched
-                                                                           else np.asarray(sv).reshape(-1)[0]),
-        mass_balance_error=float(pick(np.atleast_1d(np.asarray(err)))), provenance=prov,
+        infiltrated_volume_m3=s_b["infiltration"], stored_volume_m3=float(np.asarray(sv).reshape(-1)[b]),
+        mass_balance_error=float(np.asarray(err, dtype=np.float64).reshape(-1)[b]), provenance=prov,
         extras={"run_log": eng.log, "sources": s_b, "channel_cells": np.nonzero(eng.one.topo.kind <= 1)[0],
                 "topology": eng.one.topo, "boundary_in_m3": s_b["bnd2d_in"] + s_b["bnd1d_in"],
                 "boundary_out_m3": s_b["bnd2d_out"] + s_b["bnd1d_out"], **extras})

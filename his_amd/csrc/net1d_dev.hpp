@@ -1,0 +1,160 @@
+// Device helpers of the 1-D network and the weir exchange, shared by the network kernels and the
+// engine's fused step head: trapezoid-and-slot sections (hydrointel/solver/swe1d.py:50-84) and
+// the per-item arithmetic of the exchange (hydrointel/solver/coupling.py:73-124).
+#pragma once
+#include "net1d.hpp"
+
+namespace his {
+
+struct Sec { double b, m, yb, ts; };
+
+__device__ __forceinline__ double bank_area(const Sec& s) { return s.yb * (s.b + s.m * s.yb); }
+__device__ __forceinline__ double sec_area(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    double at = yt * (s.b + s.m * yt);
+    return at + s.ts * clamp0(y - s.yb);
+}
+__device__ __forceinline__ double sec_top(const Sec& s, double y) { return y <= s.yb ? s.b + 2.0 * s.m * y : s.ts; }
+__device__ __forceinline__ double sec_i1(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    double above = clamp0(y - s.yb);
+    double i_t = s.b * yt * yt / 2.0 + s.m * (yt * yt * yt) / 3.0;
+    double at = yt * (s.b + s.m * yt);
+    return i_t + at * above + s.ts * above * above / 2.0;
+}
+__device__ __forceinline__ double sec_perim(const Sec& s, double y) {
+    double yt = tmin(y, s.yb);
+    return s.b + 2.0 * yt * sqrt(1.0 + s.m * s.m);
+}
+__device__ __forceinline__ double sec_depth(const Sec& s, double A) {
+    A = clamp0(A);
+    double ab = bank_area(s);
+    double y_t = 2.0 * A / (s.b + sqrt(s.b * s.b + 4.0 * s.m * A));
+    return A <= ab ? y_t : s.yb + (A - ab) / s.ts;
+}
+__device__ __forceinline__ Sec cell_sec(const his_net1d& N, int b, int c) {
+    const int64_t o = (int64_t)b * N.bs + c;
+    return Sec{N.b[o], N.m[c], N.yb[o], N.ts[o]};
+}
+__device__ __forceinline__ Sec face_sec(const his_net1d& N, int b, int f) {
+    const int64_t o = (int64_t)b * N.fbs + f;
+    return Sec{N.fb[o], N.fm[f], N.fyb[o], N.fts[o]};
+}
+__device__ __forceinline__ double vel1(const Sec& s, double A, double Q, double h_dry) {
+    double y = sec_depth(s, A);
+    return y >= h_dry ? Q / (A < 1e-12 ? 1e-12 : A) : 0.0;
+}
+
+struct ExWs {
+    double *V, *sg, *Vf, *hu_, *f2, *f1;
+};
+
+__device__ __forceinline__ ExWs ex_ws(const ExchangeArgs& a, int b) {
+    const int nl = a.L.nl, nu = a.L.nu, nc = a.N.nc;
+    double* w = a.ws + (int64_t)b * (3 * nl + 2 * nu + nc);
+    return ExWs{w, w + nl, w + 2 * nl, w + 3 * nl, w + 3 * nl + nu, w + 3 * nl + 2 * nu};
+}
+
+__device__ __forceinline__ double ex_final(const ExWs& w, const his_links& L, int l) {
+    const double v = w.sg[l] > 0.0 ? w.V[l] * w.f2[L.inv[l]] : w.V[l] * w.f1[L.cell1d[l]];
+    return v * w.sg[l];
+}
+
+// Exchange phase 1 for member b, item r: link r (r < nl) -> free-weir volume capped by the
+// level-equalising volume, and its sign; unique 2-D cell r - nl -> its donor depth.
+template <typename R2>
+__device__ __forceinline__ void ex_link_item(const ExchangeArgs& a, const int b, const int r, const double dt) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    const R2* h = reinterpret_cast<const R2*>(a.h) + (int64_t)b * a.plane;
+    ExWs w = ex_ws(a, b);
+    if (r >= L.nl) {
+        const int k = r - L.nl;
+        w.hu_[k] = (double)h[L.uniq[k]];
+        return;
+    }
+    const int l = r;
+    const int c1 = L.cell1d[l];
+    const Sec s = cell_sec(N, b, c1);
+    const double h2 = (double)h[L.uniq[L.inv[l]]];
+    const double eta2 = L.z2[l] + h2;
+    const double y1 = sec_depth(s, a.A[(int64_t)b * N.nc + c1]);
+    const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
+    const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
+    const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
+    const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
+    const double q = L.weir_cw * L.length[l] * (d_up * sqrt(d_up)) * pow(clamp0(1.0 - ratio * sqrt(ratio)), 0.385);
+    w.sg[l] = eta2 > eta1 ? 1.0 : -1.0;
+    const double v = q * dt;
+    const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
+    const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
+    w.V[l] = tmin(v, L.relax * v_eq);
+}
+
+// Exchange phase 2: the donor limit factor of unique 2-D cell r (r < nu) or of 1-D cell r - nu.
+__device__ __forceinline__ void ex_limit_item(const ExchangeArgs& a, const int b, const int r) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    ExWs w = ex_ws(a, b);
+    if (r < L.nu) {
+        const int k = r;
+        double tot = 0.0;
+        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
+            const int l = L.u_link[p];
+            tot += w.sg[l] > 0.0 ? w.V[l] : 0.0;
+        }
+        const double av = L.limit * w.hu_[k] * L.area2;
+        w.f2[k] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+        return;
+    }
+    const int c = r - L.nu;
+    double tot = 0.0;
+    for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
+        const int l = L.c_link[p];
+        tot += w.sg[l] < 0.0 ? w.V[l] : 0.0;
+    }
+    const Sec s = cell_sec(N, b, c);
+    const double av = L.limit * clamp0(a.A[(int64_t)b * N.nc + c] - bank_area(s)) * N.dx[c];
+    w.f1[c] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+}
+
+// Exchange phase 3: net volume of unique 2-D cell r (r < nu) -> depth, momentum kept by
+// min(h_new / h, 1); of 1-D cell r - nu -> area.  Sums run over the target's links in link order.
+template <typename R2>
+__device__ __forceinline__ void ex_apply_item(const ExchangeArgs& a, const int b, const int r) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    ExWs w = ex_ws(a, b);
+    if (r < L.nu) {
+        const int k = r;
+        double dV = 0.0;
+        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
+            const int l = L.u_link[p];
+            const double v = ex_final(w, L, l);
+            w.Vf[l] = v;
+            if (a.V) a.V[(int64_t)b * L.nl + l] = v;
+            dV += v;
+        }
+        const double h0 = w.hu_[k];
+        const double hn = h0 - dV / L.area2;
+        const double keep = h0 > 0.0 ? tmin(hn / (h0 > 0.0 ? h0 : 1.0), 1.0) : 0.0;
+        const int64_t q = (int64_t)b * a.plane + L.uniq[k];
+        R2* h = reinterpret_cast<R2*>(a.h);
+        R2* hu = reinterpret_cast<R2*>(a.hu);
+        R2* hv = reinterpret_cast<R2*>(a.hv);
+        const R2 kp = (R2)keep;
+        h[q] = (R2)hn;
+        hu[q] = hu[q] * kp;
+        hv[q] = hv[q] * kp;
+        return;
+    }
+    const int c = r - L.nu;
+    const int p0 = L.c_ptr[c], p1 = L.c_ptr[c + 1];
+    if (p0 == p1) return;
+    double add = 0.0;
+    for (int p = p0; p < p1; ++p) add += ex_final(w, L, L.c_link[p]);
+    double* A = a.A + (int64_t)b * N.nc;
+    A[c] = A[c] + add / N.dx[c];
+}
+
+}  // namespace his
