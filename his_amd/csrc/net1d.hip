@@ -98,6 +98,9 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
     }
+    // CFL rate of the final state (swe1d.py:253-262), formed in the last substep's update
+    const bool want_rate = d.rate || d.wet;
+    double r_mx = 0.0, r_wet = 0.0;
     for (int k = 0; k < nsub; ++k) {
         double q_in, eta;
         if (d.engine) {
@@ -223,6 +226,13 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             if (!((y1 >= h_dry) && kd != 1)) Q1 = 0.0;
             A[c] = A1;
             Q[c] = Q1;
+            if (want_rate && k == nsub - 1 && y1 > h_dry) {
+                r_wet = 1.0;
+                const double cc = sqrt(g * A1 / sec_top(s, y1));
+                const double uu = vel1(s, A1, Q1, h_dry);
+                const double r = (fabs(uu) + cc) / N.dx[c];
+                r_mx = r > r_mx ? r : r_mx;
+            }
         }
         block_sum_bcast<NT, 5>(part, red);
         if (threadIdx.x == 0) {
@@ -239,7 +249,22 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
     }
-    if (d.rate || d.wet) rate1d<NT>(N, b, A, Q, d.rate, d.wet, m, red);
+    if (want_rate) {
+        r_mx = wave_max(r_mx);
+        r_wet = wave_max(r_wet);
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        if (lane == 0) { red[w][0] = r_mx; red[w][1] = r_wet; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, bw = 0.0;
+            for (int w2 = 0; w2 < NT / 64; ++w2) {
+                a = red[w2][0] > a ? red[w2][0] : a;
+                bw = red[w2][1] > bw ? red[w2][1] : bw;
+            }
+            if (d.rate) d.rate[m] = a;
+            if (d.wet) d.wet[m] = bw > 0.0 ? 1 : 0;
+        }
+    }
     if (d.engine && d.t_out && threadIdx.x == 0) {
         d.t_out[m] = t0 + dt_full;
         d.steps[m] += 1;
