@@ -39,49 +39,53 @@ int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
-    __shared__ double red[NT / 64][24];
+    // per thread: its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22) and rate maximum (23);
+    // transposed through LDS so that thread k sums quantity k over the threads in a fixed order
+    constexpr int NV = 24;
+    __shared__ double tr[NT][NV + 1];
+    __shared__ double tot[NV];
     const int m = blockIdx.x;
-    if (F.active && !F.active[m]) return;
+    if (F.snap) {
+        if (F.snap[2 * F.Btot + m] == 0.0) return;     // the member did not step
+    } else if (F.active && !F.active[m]) {
+        return;
+    }
     if (F.only && !F.only[m]) return;
     int gi = 0;
     while (gi + 1 < F.ngroups && m >= F.member0[gi + 1]) ++gi;
     const int b = m - F.member0[gi];
     const int per = F.per[gi];
     const int64_t t0 = F.tile0[gi] + (int64_t)b * per;
-    double s1[P_NOUT + 1], s2[P_USED - 1];
-    for (int k = 0; k <= P_NOUT; ++k) s1[k] = 0.0;
-    for (int k = 0; k < P_USED - 1; ++k) s2[k] = 0.0;
-    double rmax = 0.0;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = 0.0;
     for (int t = threadIdx.x; t < per; t += NT) {
         const double* p = F.part + (t0 + t) * NP2;
         if (!F.stage_only)
-            for (int k = 0; k <= P_NOUT; ++k) s1[k] += p[k];
-        for (int k = 0; k < P_USED - 1; ++k) s2[k] += p[NP + k];
-        rmax = p[NP + P_RATE] > rmax ? p[NP + P_RATE] : rmax;
+#pragma unroll
+            for (int k = 0; k <= P_NOUT; ++k) v[k] += p[k];
+#pragma unroll
+        for (int k = 0; k < P_USED - 1; ++k) v[10 + k] += p[NP + k];
+        v[23] = p[NP + P_RATE] > v[23] ? p[NP + P_RATE] : v[23];
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = 0; k <= P_NOUT; ++k) s1[k] = wave_sum(s1[k]);
-    for (int k = 0; k < P_USED - 1; ++k) s2[k] = wave_sum(s2[k]);
-    rmax = wave_max(rmax);
-    if (lane == 0) {
-        for (int k = 0; k <= P_NOUT; ++k) red[w][k] = s1[k];
-        for (int k = 0; k < P_USED - 1; ++k) red[w][10 + k] = s2[k];
-        red[w][23] = rmax;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tr[threadIdx.x][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        double a = 0.0;
+        if (k == 23)
+            for (int w = 0; w < NT; ++w) a = tr[w][k] > a ? tr[w][k] : a;
+        else
+            for (int w = 0; w < NT; ++w) a += tr[w][k];
+        tot[k] = a;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int k = 0; k <= P_NOUT; ++k) {
-        double s = 0.0;
-        for (int w2 = 0; w2 < NT / 64; ++w2) s += red[w2][k];
-        s1[k] = s;
-    }
-    for (int k = 0; k < P_USED - 1; ++k) {
-        double s = 0.0;
-        for (int w2 = 0; w2 < NT / 64; ++w2) s += red[w2][10 + k];
-        s2[k] = s;
-    }
-    rmax = 0.0;
-    for (int w2 = 0; w2 < NT / 64; ++w2) rmax = red[w2][23] > rmax ? red[w2][23] : rmax;
+    double s1[P_NOUT + 1], s2[P_USED - 1];
+    for (int k = 0; k <= P_NOUT; ++k) s1[k] = tot[k];
+    for (int k = 0; k < P_USED - 1; ++k) s2[k] = tot[10 + k];
+    const double rmax = tot[23];
     const double area = F.area[gi];
     if (F.rate) F.rate[m] = rmax;
     if (F.volume) {
@@ -89,7 +93,7 @@ __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
         F.volume[F.Btot + m] = s2[P_VOLS] * area;
     }
     if (F.stage_only) return;
-    const double dt = F.dt[m], dx = F.dx[gi], dy = F.dy[gi];
+    const double dt = F.snap ? F.snap[m] : F.dt[m], dx = F.dx[gi], dy = F.dy[gi];
     const double q2i = (s2[P_WIN] + s2[P_EIN]) * dy + (s2[P_SIN] + s2[P_NIN]) * dx;
     const double q2o = (s2[P_WOUT] + s2[P_EOUT]) * dy + (s2[P_SOUT] + s2[P_NOUT]) * dx;
     double qin, qout, clip;
@@ -104,7 +108,7 @@ __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
         qout = q2o;
         clip = s2[P_CLIP] * area;
     }
-    const double rain = F.pm[m];
+    const double rain = F.snap ? F.snap[F.Btot + m] : F.pm[m];
     double rain_v;
     if (F.weighted[gi]) rain_v = s2[P_RAINW] * area;
     else {
@@ -131,6 +135,7 @@ __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
         F.acc[3 * F.Btot + m] += bout;
         F.acc[6 * F.Btot + m] += clip;
     }
+    if (F.snap) F.snap[2 * F.Btot + m] = 0.0;
 }
 
 int launch_finalize(const FinalizeArgs& f, hipStream_t s) {
@@ -420,6 +425,7 @@ struct DtArgs {
     const double* rate2d;      // (Btot)
     const double* rate1d;      // (Btot)
     const int32_t* wet1d;      // (Btot)
+    double* rate2d_reset;      // (Btot) or NULL: cleared for every member that steps, for the stage-2 atomic max
     int Btot;
 };
 
@@ -502,6 +508,7 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
         a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
         a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
         a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
+        if (a.rate2d_reset) a.rate2d_reset[m] = 0.0;
     }
     if (!act) return;
     // the forcing of the step, one (member, slot) per thread: rain at t, then W, E, S, N at t and t + dt
@@ -539,16 +546,18 @@ struct his_engine {
     double* ws1d[HIS_MAX_GROUPS];
     double* wsx[HIS_MAX_GROUPS];
     int32_t* active;
-    double *pm, *rate2d, *rate1d, *volume, *t_step;
+    double *pm, *rate2d, *rate1d, *volume, *t_step, *snap;
     int32_t* wet1d;
-    hipStream_t cap, side;
-    hipEvent_t ev_fork, ev_join;
+    hipStream_t cap, side, fin;
+    hipEvent_t ev_fork, ev_join, ev_s2, ev_fin;
+    int async_fin;            // ledger reduction on its own stream, beside the next step's head
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
     int kmulti;
     int graphs_ready;
 };
 
-static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
+// fin_pending: a ledger reduction launched on e->fin has not yet been waited for on s
+static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, bool& fin_pending) {
     int rc;
     hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(e->n_dtg == 1 ? 256 : 64), 0, s, e->D);
     HIS_HIP(hipGetLastError());
@@ -586,9 +595,22 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side) {
         HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
     }
     if ((rc = launch_swe1d_groups(W, s1))) return rc;
+    if (fin_pending) {                         // the previous step's partials must be reduced first
+        HIS_HIP(hipStreamWaitEvent(s, e->ev_fin, 0));
+        fin_pending = false;
+    }
     rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
     if (rc) return rc;
-    if ((rc = launch_finalize(e->F, s))) return rc;
+    if (e->async_fin) {
+        // the next step's dt reads the rates the last stage folded in; only the ledger waits here
+        HIS_HIP(hipEventRecord(e->ev_s2, s));
+        HIS_HIP(hipStreamWaitEvent(e->fin, e->ev_s2, 0));
+        if ((rc = launch_finalize(e->F, e->fin))) return rc;
+        HIS_HIP(hipEventRecord(e->ev_fin, e->fin));
+        fin_pending = true;
+    } else if ((rc = launch_finalize(e->F, s))) {
+        return rc;
+    }
     if (any_net && side) {
         HIS_HIP(hipEventRecord(e->ev_join, side));
         HIS_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
@@ -614,7 +636,8 @@ extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
     }
     tot += align_up((size_t)tiles * NP2 * 8);
     tot += align_up((size_t)Btot * 4) + align_up((size_t)Btot * 9 * 8) + 3 * align_up((size_t)Btot * 8) +
-           align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * sizeof(DtGroup));
+           align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * 3 * 8) +
+           align_up((size_t)Btot * sizeof(DtGroup));
     return tot;
 }
 
@@ -704,16 +727,27 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     e->t_step = (double*)p; p += align_up((size_t)Btot * 8);
     e->volume = (double*)p; p += align_up((size_t)Btot * 2 * 8);
     e->wet1d = (int32_t*)p; p += align_up((size_t)Btot * 4);
+    e->snap = (double*)p; p += align_up((size_t)Btot * 3 * 8);
     DtGroup* dtg_dev = (DtGroup*)p;
-    // launches
+    // launches; with a network in every group the 1-D kernel advances t, so the ledger reduction is
+    // off the step's critical path and the last stage publishes the CFL rates itself
+    e->async_fin = 1;
+    for (int gi = 0; gi < d->ngroups; ++gi) e->async_fin &= e->has_net[gi];
+    unsigned long long* rate_bits = e->async_fin ? (unsigned long long*)e->rate2d : nullptr;
     if (dtype == HIS_F64) {
         e->L64.dt = c.dt; e->L64.t = e->t_step; e->L64.pm = e->pm; e->L64.active = e->active; e->L64.part = part;
+        e->L64.rate = rate_bits;
+        e->L64.snap = e->async_fin ? e->snap : nullptr;
     } else {
         e->L32.dt = c.dt; e->L32.t = e->t_step; e->L32.pm = e->pm; e->L32.active = e->active; e->L32.part = part;
+        e->L32.rate = rate_bits;
+        e->L32.snap = e->async_fin ? e->snap : nullptr;
     }
     FinalizeArgs& F = e->F;
     F = FinalizeArgs{};
-    F.part = part; F.dt = c.dt; F.pm = e->pm; F.active = e->active; F.acc = c.acc; F.rate = e->rate2d;
+    F.part = part; F.dt = c.dt; F.pm = e->pm; F.active = e->active; F.acc = c.acc;
+    F.rate = e->async_fin ? nullptr : e->rate2d;
+    F.snap = e->async_fin ? e->snap : nullptr;
     F.volume = e->volume; F.Btot = Btot; F.ngroups = d->ngroups;
     F.t_out = c.t; F.t_step = e->t_step; F.steps = c.steps;
     std::vector<DtGroup> dtg;
@@ -732,6 +766,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     D = DtArgs{};
     D.groups = dtg_dev; D.ngroups_dt = e->n_dtg; D.cfl = d->cfl; D.dt_min = d->dt_min; D.dt_max = d->dt_max;
     D.ctl = c; D.active = e->active; D.t_step = e->t_step; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
+    D.rate2d_reset = e->async_fin ? e->rate2d : nullptr;
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];
@@ -745,12 +780,17 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     hipStream_t s = (hipStream_t)stream;
     hipError_t he = hipMemcpyAsync(dtg_dev, dtg.data(), dtg.size() * sizeof(DtGroup), hipMemcpyHostToDevice, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(dt groups)"); }
+    he = hipMemsetAsync(e->snap, 0, (size_t)Btot * 3 * 8, s);
+    if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemsetAsync(ledger snapshot)"); }
     he = hipStreamSynchronize(s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipStreamSynchronize"); }
     if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->fin, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_s2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->ev_fin, hipEventDisableTiming) != hipSuccess) {
         delete e;
         set_error("his_engine_create: stream/event creation failed");
         return HIS_E_HIP;
@@ -771,6 +811,8 @@ static int engine_refresh(his_engine* e, const int32_t* mask, hipStream_t s) {
     F.active = nullptr;
     F.acc = nullptr;
     F.only = mask;
+    F.rate = e->rate2d;
+    F.snap = nullptr;
     if ((rc = launch_finalize(F, s))) return rc;
     for (int gi = 0; gi < e->d.ngroups; ++gi) {
         if (!e->has_net[gi]) continue;
@@ -798,7 +840,12 @@ static int engine_build_graphs(his_engine* e) {
         hipGraph_t g;
         HIS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));
         int rc = 0;
-        for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side);
+        bool pend = false;
+        for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side, pend);
+        if (!rc && pend) {
+            hipError_t we = hipStreamWaitEvent(e->cap, e->ev_fin, 0);
+            if (we != hipSuccess) rc = hip_fail(we, "hipStreamWaitEvent");
+        }
         hipError_t he = hipStreamEndCapture(e->cap, &g);
         if (rc) return rc;
         if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture");
@@ -814,10 +861,12 @@ extern "C" int his_engine_step(his_engine* e, int32_t n_steps, void* stream) {
     if (!e || n_steps < 0) { set_error("his_engine_step: bad argument"); return HIS_E_ARG; }
     hipStream_t s = (hipStream_t)stream;
     if (!e->d.use_graph) {
+        bool pend = false;
         for (int i = 0; i < n_steps; ++i) {
-            int rc = engine_enqueue_step(e, s, e->side);
+            int rc = engine_enqueue_step(e, s, e->side, pend);
             if (rc) return rc;
         }
+        if (pend) HIS_HIP(hipStreamWaitEvent(s, e->ev_fin, 0));
         return 0;
     }
     if (!e->graphs_ready) {
@@ -848,8 +897,11 @@ extern "C" int his_engine_destroy(his_engine* e) {
     }
     (void)hipStreamDestroy(e->cap);
     (void)hipStreamDestroy(e->side);
+    (void)hipStreamDestroy(e->fin);
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
+    (void)hipEventDestroy(e->ev_s2);
+    (void)hipEventDestroy(e->ev_fin);
     delete e;
     return 0;
 }

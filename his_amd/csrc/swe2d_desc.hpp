@@ -39,6 +39,13 @@ struct Launch2D {
     const double *dt, *t, *pm;    // (Btot), (Btot), (9, Btot)
     const int32_t *active;        // (Btot) or NULL = every member steps
     double *part;                 // (ntiles, NP2)
+    // (Btot) or NULL: the last stage also folds each tile's CFL-rate maximum into rate[m] with an
+    // atomic max on the bit pattern (non-negative doubles order as unsigned integers), so the next
+    // step's dt does not wait for the partials to be reduced
+    unsigned long long *rate;
+    // (3, Btot) or NULL: the last stage's first tile of each member records that member's dt, rain
+    // rate and a 1 (stepped) for a ledger reduction that runs while the next step's dt is chosen
+    double *snap;
 };
 
 // Kernel shapes compiled for each precision; the index into this table is the tuning knob.
@@ -86,6 +93,8 @@ struct FinalizeArgs {
     int rain_f32[HIS_MAX_GROUPS];        // rain * dt formed in float32 (the working dtype) before widening
     int stage_only;                      // 1 = only rate/volume (after a state reduction)
     const int32_t* only;                 // stage_only: members with only[m] == 0 are left untouched
+    double* snap;                        // (3, Btot) or NULL: dt, rain and stepped flag of the step reduced
+                                         // (recorded by the last stage; the flag is cleared here)
     // engine: members of groups without a 1-D network advance here, t_out = t_step + dt, steps += 1
     double* t_out;
     const double* t_step;
