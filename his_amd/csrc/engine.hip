@@ -13,7 +13,7 @@
 #include <cstring>
 #include <vector>
 
-#include "net1d.hpp"
+#include "step_head.hpp"
 #include "swe2d_desc.hpp"
 
 namespace his {
@@ -406,128 +406,18 @@ extern "C" int his_exchange(const his_links* l, const his_net1d* n, int32_t dtyp
 
 namespace his {
 
-struct DtGroup { int m0, m1, gi, pad; };
-
-struct DtArgs {
-    const DtGroup* groups;
-    int ngroups_dt;
-    double cfl, dt_min, dt_max;
-    // per grid group
-    int bc[HIS_MAX_GROUPS][4];
-    his_series rain[HIS_MAX_GROUPS], edge[HIS_MAX_GROUPS][4];
-    int member0[HIS_MAX_GROUPS];
-    int has_net[HIS_MAX_GROUPS];
-    double cfl_1d[HIS_MAX_GROUPS], dt_max_1d[HIS_MAX_GROUPS];
-    his_control ctl;
-    int32_t* active;
-    double* t_step;            // (Btot) start time of the current step
-    double* pm;                // (9, Btot)
-    const double* rate2d;      // (Btot)
-    const double* rate1d;      // (Btot)
-    const int32_t* wet1d;      // (Btot)
-    double* rate2d_reset;      // (Btot) or NULL: cleared for every member that steps, for the stage-2 atomic max
-    int Btot;
-};
-
 // One block per shared-dt group: choose dt and n_sub, mark members active, evaluate the forcing.
 // Snapshots each member's step start time into t_step, which the step's kernels read while the
 // 1-D kernel (or finalize, without a network) advances t.
 __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
     const DtGroup G = a.groups[blockIdx.x];
-    const int gi = G.gi;
     __shared__ double red[4][3];
-    __shared__ double sh_dt;
-    __shared__ int sh_act, sh_nsub;
-    double r2 = 0.0, r1 = 0.0, w1 = 0.0;
-    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
-        r2 = a.rate2d[m] > r2 ? a.rate2d[m] : r2;
-        if (a.has_net[gi]) {
-            r1 = a.rate1d[m] > r1 ? a.rate1d[m] : r1;
-            w1 = a.wet1d[m] ? 1.0 : w1;
-        }
-    }
-    r2 = wave_max(r2);
-    r1 = wave_max(r1);
-    w1 = wave_max(w1);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { red[w][0] = r2; red[w][1] = r1; red[w][2] = w1; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
-            r2 = red[k][0] > r2 ? red[k][0] : r2;
-            r1 = red[k][1] > r1 ? red[k][1] : r1;
-            w1 = red[k][2] > w1 ? red[k][2] : w1;
-        }
-        const int m0 = G.m0;
-        const double t = a.ctl.t[m0];
-        const double lim = a.ctl.t_limit[m0] < a.ctl.t_end[m0] ? a.ctl.t_limit[m0] : a.ctl.t_end[m0];
-        const int act = t < lim - 1e-9;
-        double dt = 0.0;
-        int nsub = 0, clamp = 0, err = 0;
-        if (act && !(r2 == r2 && r2 < 1e300 && r1 == r1 && r1 < 1e300)) err = 2;   // NaN or inf rate
-        if (act && !err) {
-            double d2 = r2 > 0.0 ? a.cfl / r2 : a.dt_max;
-            if (d2 < a.dt_min || d2 > a.dt_max) {
-                clamp = d2 < a.dt_min;
-                d2 = d2 < a.dt_min ? a.dt_min : d2;
-                d2 = d2 > a.dt_max ? a.dt_max : d2;
-            }
-            dt = d2 < lim - t ? d2 : lim - t;
-            if (a.has_net[gi]) {
-                double d1 = a.dt_max_1d[gi];
-                if (w1 > 0.0) {
-                    d1 = a.cfl_1d[gi] / r1;
-                    d1 = d1 < a.dt_max_1d[gi] ? d1 : a.dt_max_1d[gi];
-                }
-                const double q = ceil(dt / d1 - 1e-9);
-                if (q > (double)a.ctl.n_sub_limit) err = 1;
-                else nsub = q < 1.0 ? 1 : (int)q;
-            }
-        }
-        if (err) {
-            for (int m = G.m0; m < G.m1; ++m) a.ctl.error[m] = err;
-        }
-        sh_dt = dt;
-        sh_act = act && !err && a.ctl.error[m0] == 0;
-        sh_nsub = nsub;
-        red[0][0] = clamp;
-    }
-    __syncthreads();
-    const int act = sh_act;
-    const double dt = sh_dt;
-    const int nsub = sh_nsub;
-    const int clamp = red[0][0] != 0.0;
-    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
-        a.active[m] = act;
-        if (!act) continue;
-        const double t = a.ctl.t[m];
-        a.t_step[m] = t;
-        a.ctl.dt[m] = dt;
-        a.ctl.n_sub[m] = nsub;
-        if (clamp) a.ctl.clamped[m] += 1;
-        a.ctl.dt_min_seen[m] = dt < a.ctl.dt_min_seen[m] ? dt : a.ctl.dt_min_seen[m];
-        a.ctl.dt_max_seen[m] = dt > a.ctl.dt_max_seen[m] ? dt : a.ctl.dt_max_seen[m];
-        a.ctl.n_sub_max[m] = nsub > a.ctl.n_sub_max[m] ? nsub : a.ctl.n_sub_max[m];
-        if (a.rate2d_reset) a.rate2d_reset[m] = 0.0;
-    }
-    if (!act) return;
-    // the forcing of the step, one (member, slot) per thread: rain at t, then W, E, S, N at t and t + dt
+    __shared__ HeadState hs;
+    head_group_dt(a, G, red, hs);
+    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) head_member_ctl(a, m, hs, a.rate2d_reset);
+    if (!hs.go) return;
     const int nm = G.m1 - G.m0;
-    for (int k = threadIdx.x; k < nm * 9; k += blockDim.x) {
-        const int mi = k / 9, slot = k - mi * 9;
-        const int m = G.m0 + mi;
-        const int b = m - a.member0[gi];
-        const double t = a.ctl.t[m];
-        double v = 0.0;
-        if (slot == 0) {
-            v = series_eval(a.rain[gi], b, t);
-        } else {
-            const int e = (slot - 1) & 3;
-            const bool valued = a.bc[gi][e] == HIS_BC_STAGE || a.bc[gi][e] == HIS_BC_DISCHARGE;
-            if (valued) v = series_eval(a.edge[gi][e], b, slot <= 4 ? t : t + dt);
-        }
-        a.pm[slot * a.Btot + m] = v;
-    }
+    for (int k = threadIdx.x; k < nm * 9; k += blockDim.x) head_member_pm(a, G.gi, G.m0 + k / 9, k % 9, hs.dt);
 }
 
 }  // namespace his
@@ -551,6 +441,9 @@ struct his_engine {
     hipStream_t cap, side, fin;
     hipEvent_t ev_fork, ev_join, ev_s2, ev_fin;
     int async_fin;            // ledger reduction on its own stream, beside the next step's head
+    int fuse_head;            // dt and forcing evaluated inside the first exchange phase
+    DtArgs* head_dev;         // device copy of D for the fused head
+    int32_t* dtg_of;          // (Btot) device: dt group of each member
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
     int kmulti;
     int graphs_ready;
@@ -559,8 +452,10 @@ struct his_engine {
 // fin_pending: a ledger reduction launched on e->fin has not yet been waited for on s
 static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, bool& fin_pending) {
     int rc;
-    hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(e->n_dtg == 1 ? 256 : 64), 0, s, e->D);
-    HIS_HIP(hipGetLastError());
+    if (!e->fuse_head) {
+        hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(e->n_dtg == 1 ? 256 : 64), 0, s, e->D);
+        HIS_HIP(hipGetLastError());
+    }
     // every group's weir exchange: one launch per phase
     ExGroups X{};
     Swe1DGroups W{};
@@ -585,6 +480,11 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, b
         d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
         d.member0 = m0;
+    }
+    if (e->fuse_head) {
+        X.head = e->head_dev;
+        X.dtg_of = e->dtg_of;
+        X.rate_reset = e->rate2d;
     }
     if ((rc = launch_exchange_groups(X, s))) return rc;
     // every group's 1-D network in one launch, on a side stream beside the 2-D stages
@@ -637,7 +537,7 @@ extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
     tot += align_up((size_t)tiles * NP2 * 8);
     tot += align_up((size_t)Btot * 4) + align_up((size_t)Btot * 9 * 8) + 3 * align_up((size_t)Btot * 8) +
            align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * 3 * 8) +
-           align_up((size_t)Btot * sizeof(DtGroup));
+           align_up((size_t)Btot * sizeof(DtGroup)) + align_up(sizeof(DtArgs)) + align_up((size_t)Btot * 4);
     return tot;
 }
 
@@ -728,7 +628,9 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     e->volume = (double*)p; p += align_up((size_t)Btot * 2 * 8);
     e->wet1d = (int32_t*)p; p += align_up((size_t)Btot * 4);
     e->snap = (double*)p; p += align_up((size_t)Btot * 3 * 8);
-    DtGroup* dtg_dev = (DtGroup*)p;
+    DtGroup* dtg_dev = (DtGroup*)p; p += align_up((size_t)Btot * sizeof(DtGroup));
+    e->head_dev = (DtArgs*)p; p += align_up(sizeof(DtArgs));
+    e->dtg_of = (int32_t*)p;
     // launches; with a network in every group the 1-D kernel advances t, so the ledger reduction is
     // off the step's critical path and the last stage publishes the CFL rates itself
     e->async_fin = 1;
@@ -766,7 +668,11 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     D = DtArgs{};
     D.groups = dtg_dev; D.ngroups_dt = e->n_dtg; D.cfl = d->cfl; D.dt_min = d->dt_min; D.dt_max = d->dt_max;
     D.ctl = c; D.active = e->active; D.t_step = e->t_step; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
-    D.rate2d_reset = e->async_fin ? e->rate2d : nullptr;
+    // the fused head needs every member's links to span a whole block (two members per block at most)
+    e->fuse_head = e->async_fin;
+    for (int gi = 0; gi < d->ngroups; ++gi)
+        if (e->has_net[gi] && e->links[gi].nl + e->links[gi].nu < 256) e->fuse_head = 0;
+    D.rate2d_reset = e->async_fin && !e->fuse_head ? e->rate2d : nullptr;
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];
@@ -780,6 +686,12 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     hipStream_t s = (hipStream_t)stream;
     hipError_t he = hipMemcpyAsync(dtg_dev, dtg.data(), dtg.size() * sizeof(DtGroup), hipMemcpyHostToDevice, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(dt groups)"); }
+    std::vector<int32_t> dtg_of(Btot, 0);
+    for (int k = 0; k < (int)dtg.size(); ++k)
+        for (int m = dtg[k].m0; m < dtg[k].m1; ++m) dtg_of[m] = k;
+    he = hipMemcpyAsync(e->head_dev, &e->D, sizeof(DtArgs), hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(e->dtg_of, dtg_of.data(), (size_t)Btot * 4, hipMemcpyHostToDevice, s);
+    if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(step head)"); }
     he = hipMemsetAsync(e->snap, 0, (size_t)Btot * 3 * 8, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemsetAsync(ledger snapshot)"); }
     he = hipStreamSynchronize(s);

@@ -9,7 +9,7 @@
 // SWE1D._step_core  hydrointel/solver/swe1d.py:320-368 (+ _fill_ghosts 265-294,
 //                   Sections 50-84, hll_1d 87-107, max_dt 253-262)
 // exchange          hydrointel/solver/coupling.py:73-124
-#include "net1d_dev.hpp"
+#include "step_head.hpp"
 
 namespace his {
 
@@ -340,18 +340,61 @@ __device__ __forceinline__ int ex_group(const ExGroups& G, const int64_t* it0, i
     return gi;
 }
 
+// member of link-phase item it, and whether it is that member's first item
+__device__ __forceinline__ int ex_link_member(const ExGroups& G, int64_t it, bool& first) {
+    const int gi = ex_group(G, G.it_links, it);
+    const ExchangeArgs& a = G.a[gi];
+    const int64_t local = it - G.it_links[gi];
+    const int per = a.L.nl + a.L.nu;
+    const int b = (int)(local / per);
+    first = local == (int64_t)b * per;
+    return a.member0 + b;
+}
+
 template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
+    __shared__ double red[4][3];
+    __shared__ HeadState hs[2];
+    const int64_t total = G.it_links[G.ngroups];
+    int mA = -1, mB = -1;
+    if (G.head) {
+        // a block spans at most two members (nl + nu >= 256): evaluate their groups' dt here
+        const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+        const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
+        bool firstA, firstB;
+        mA = ex_link_member(G, i0, firstA);
+        mB = ex_link_member(G, i1, firstB);
+        head_group_dt(*G.head, G.head->groups[G.dtg_of[mA]], red, hs[0]);
+        if (mB != mA) head_group_dt(*G.head, G.head->groups[G.dtg_of[mB]], red, hs[1]);
+        // the block holding a member's first item writes its control entries and its forcing
+        for (int j = 0; j < 2; ++j) {
+            const int mm = j == 0 ? mA : mB;
+            const bool mine = j == 0 ? firstA : (mB != mA);
+            if (!mine) continue;
+            const HeadState& h = hs[j];
+            const int gim = G.head->groups[G.dtg_of[mm]].gi;
+            if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr);
+            if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(*G.head, gim, mm, threadIdx.x - 64, h.dt);
+        }
+    }
     int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (it >= G.it_links[G.ngroups]) return;
+    if (it >= total) return;
     const int gi = ex_group(G, G.it_links, it);
     const ExchangeArgs& a = G.a[gi];
     it -= G.it_links[gi];
     const int per = a.L.nl + a.L.nu;
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
-    if (a.active && !a.active[m]) return;
-    ex_link_item<R2>(a, b, r, a.dt[m]);
+    double dt;
+    if (G.head) {
+        const HeadState& h = m == mA ? hs[0] : hs[1];
+        if (!h.go) return;
+        dt = h.dt;
+    } else {
+        if (a.active && !a.active[m]) return;
+        dt = a.dt[m];
+    }
+    ex_link_item<R2>(a, b, r, dt);
 }
 
 __global__ void __launch_bounds__(256) k_ex_limits(const ExGroups G) {
@@ -378,6 +421,7 @@ __global__ void __launch_bounds__(256) k_ex_apply(const ExGroups G) {
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     if (a.active && !a.active[m]) return;
+    if (G.rate_reset && r == 0) G.rate_reset[m] = 0.0;   // every dt of this step has been taken
     ex_apply_item<R2>(a, b, r);
 }
 

@@ -52,11 +52,19 @@ struct ExchangeArgs {
 };
 int launch_exchange(const ExchangeArgs& a, hipStream_t s);
 
+struct DtArgs;
+
 // Several groups' exchanges in one launch per phase (every group shares the 2-D dtype).
 struct ExGroups {
     int ngroups;
     int64_t it_links[HIS_MAX_GROUPS + 1], it_cells[HIS_MAX_GROUPS + 1];   // filled by the launcher
     ExchangeArgs a[HIS_MAX_GROUPS];
+    // optional step head fused into the first phase (device copy of the engine's dt arguments and the
+    // dt group of every member); each block then derives its members' dt itself and the block holding
+    // a member's first link writes that member's control entries.  Needs every group's nl + nu >= 256.
+    const DtArgs* head;
+    const int32_t* dtg_of;
+    double* rate_reset;      // (Btot) or NULL: cleared in the last phase for every member that steps
 };
 int launch_exchange_groups(ExGroups& g, hipStream_t s);
 
