@@ -476,7 +476,7 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, b
         d.t_out = e->d.ctl.t; d.steps = e->d.ctl.steps;
         d.ex_vf = exchange_vf(e->wsx[gi], e->links[gi], e->net[gi], 0);
         d.ex_nl = e->links[gi].nl;
-        d.ex_stride = 3 * e->links[gi].nl + 2 * e->links[gi].nu + e->net[gi].nc;
+        d.ex_stride = (int)exchange_ws_stride(e->links[gi], e->net[gi]);
         d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
         d.member0 = m0;

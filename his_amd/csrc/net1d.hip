@@ -321,16 +321,17 @@ int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, doub
 
 // ------------------------------------------------------------------ exchange
 //
-// Four grid-wide kernels, every member and link / unique 2-D cell / 1-D cell in flight at once:
+// Three grid-wide kernels, every member and link / unique 2-D cell / 1-D cell in flight at once:
 //   1. per link: free-weir + Villemonte volume capped by the level-equalising volume, its sign;
-//      per unique 2-D cell: the donor depth h_u
-//   2. per unique 2-D cell / 1-D cell: the donor limit factor from the links' gross give
-//   3. per unique 2-D cell: net volume -> new depth, momentum kept by min(h_new/h, 1);
-//      per 1-D cell: net volume -> new area   (sums run over each target's links in link order)
-//   4. per member: gross volumes each way into the ledger (fixed-order block reduction)
+//      per unique 2-D cell: the donor depth h_u; per 1-D cell (by its first link): its available
+//      donor volume
+//   2. per unique 2-D cell: donor limit factors of its links -> net volume -> new depth, momentum
+//      kept by min(h_new/h, 1); per 1-D cell: the same -> new area (sums run over each target's
+//      links in link order; limits read only phase 1's outputs, so in-place updates do not feed them)
+//   3. per member: gross volumes each way into the ledger (fixed-order block reduction)
 
 size_t exchange_ws_doubles(const his_links& l, const his_net1d& n) {
-    return (size_t)n.B * (3 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc);
+    return (size_t)n.B * (size_t)exchange_ws_stride(l, n);
 }
 
 // thread -> (group, item within the group) for a grid that spans every group's items
@@ -397,19 +398,6 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     ex_link_item<R2>(a, b, r, dt);
 }
 
-__global__ void __launch_bounds__(256) k_ex_limits(const ExGroups G) {
-    int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (it >= G.it_cells[G.ngroups]) return;
-    const int gi = ex_group(G, G.it_cells, it);
-    const ExchangeArgs& a = G.a[gi];
-    it -= G.it_cells[gi];
-    const int per = a.L.nu + a.N.nc;
-    const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
-    const int m = a.member0 + b;
-    if (a.active && !a.active[m]) return;
-    ex_limit_item(a, b, r);
-}
-
 template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_apply(const ExGroups G) {
     int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -458,8 +446,6 @@ int launch_exchange_groups(ExGroups& G, hipStream_t s) {
     const dim3 g1 = blocks(G.it_links[G.ngroups]), g2 = blocks(G.it_cells[G.ngroups]);
     if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, G);
     else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, G);
-    HIS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_ex_limits, g2, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());
     if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, G);
     else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, G);

@@ -79,9 +79,14 @@ struct Swe1DGroups {
 int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s);
 
 size_t swe1d_ws_doubles(const his_net1d& n);
+// doubles of exchange workspace per member: V, sign, final V (nl each), donor depth of each unique
+// 2-D cell (nu), available donor volume of each 1-D cell (nc)
+__host__ __device__ inline int64_t exchange_ws_stride(const his_links& l, const his_net1d& n) {
+    return 3 * (int64_t)l.nl + (int64_t)l.nu + (int64_t)n.nc;
+}
 // signed link volumes of the last exchange for member b (float64, nl values) inside its workspace
 inline const double* exchange_vf(double* ws, const his_links& l, const his_net1d& n, int b) {
-    return ws + (size_t)b * (3 * (size_t)l.nl + 2 * (size_t)l.nu + (size_t)n.nc) + 2 * (size_t)l.nl;
+    return ws + (size_t)b * (size_t)exchange_ws_stride(l, n) + 2 * (size_t)l.nl;
 }
 size_t exchange_ws_doubles(const his_links& l, const his_net1d& n);
 

@@ -46,22 +46,42 @@ __device__ __forceinline__ double vel1(const Sec& s, double A, double Q, double 
 }
 
 struct ExWs {
-    double *V, *sg, *Vf, *hu_, *f2, *f1;
+    double *V, *sg, *Vf, *hu_, *av1;
 };
 
 __device__ __forceinline__ ExWs ex_ws(const ExchangeArgs& a, int b) {
-    const int nl = a.L.nl, nu = a.L.nu, nc = a.N.nc;
-    double* w = a.ws + (int64_t)b * (3 * nl + 2 * nu + nc);
-    return ExWs{w, w + nl, w + 2 * nl, w + 3 * nl, w + 3 * nl + nu, w + 3 * nl + 2 * nu};
+    const int nl = a.L.nl, nu = a.L.nu;
+    double* w = a.ws + (int64_t)b * exchange_ws_stride(a.L, a.N);
+    return ExWs{w, w + nl, w + 2 * nl, w + 3 * nl, w + 3 * nl + nu};
 }
 
-__device__ __forceinline__ double ex_final(const ExWs& w, const his_links& L, int l) {
-    const double v = w.sg[l] > 0.0 ? w.V[l] * w.f2[L.inv[l]] : w.V[l] * w.f1[L.cell1d[l]];
-    return v * w.sg[l];
+// donor limit factor of one target: the links' gross give `tot` scaled down to `av` when it exceeds it
+__device__ __forceinline__ double ex_factor(double tot, double av) {
+    return tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
+}
+// limit factor of unique 2-D cell k from the gross volume its links give to the 1-D side
+__device__ __forceinline__ double ex_f2(const ExWs& w, const his_links& L, int k) {
+    double tot = 0.0;
+    for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
+        const int l = L.u_link[p];
+        tot += w.sg[l] > 0.0 ? w.V[l] : 0.0;
+    }
+    return ex_factor(tot, L.limit * w.hu_[k] * L.area2);
+}
+// limit factor of 1-D cell c from the gross volume its links give to the 2-D side
+__device__ __forceinline__ double ex_f1(const ExWs& w, const his_links& L, int c) {
+    double tot = 0.0;
+    for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
+        const int l = L.c_link[p];
+        tot += w.sg[l] < 0.0 ? w.V[l] : 0.0;
+    }
+    return ex_factor(tot, w.av1[c]);
 }
 
 // Exchange phase 1 for member b, item r: link r (r < nl) -> free-weir volume capped by the
-// level-equalising volume, and its sign; unique 2-D cell r - nl -> its donor depth.
+// level-equalising volume, and its sign; the first link of each 1-D cell also stores that cell's
+// available donor volume; unique 2-D cell r - nl -> its donor depth.  Phase 2 reads only these, so
+// its in-place updates of h and A never feed another item's limit.
 template <typename R2>
 __device__ __forceinline__ void ex_link_item(const ExchangeArgs& a, const int b, const int r, const double dt) {
     const his_links& L = a.L;
@@ -78,7 +98,8 @@ __device__ __forceinline__ void ex_link_item(const ExchangeArgs& a, const int b,
     const Sec s = cell_sec(N, b, c1);
     const double h2 = (double)h[L.uniq[L.inv[l]]];
     const double eta2 = L.z2[l] + h2;
-    const double y1 = sec_depth(s, a.A[(int64_t)b * N.nc + c1]);
+    const double A1 = a.A[(int64_t)b * N.nc + c1];
+    const double y1 = sec_depth(s, A1);
     const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
     const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
     const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
@@ -89,37 +110,12 @@ __device__ __forceinline__ void ex_link_item(const ExchangeArgs& a, const int b,
     const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
     const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
     w.V[l] = tmin(v, L.relax * v_eq);
+    if (L.c_link[L.c_ptr[c1]] == l) w.av1[c1] = L.limit * clamp0(A1 - bank_area(s)) * N.dx[c1];
 }
 
-// Exchange phase 2: the donor limit factor of unique 2-D cell r (r < nu) or of 1-D cell r - nu.
-__device__ __forceinline__ void ex_limit_item(const ExchangeArgs& a, const int b, const int r) {
-    const his_links& L = a.L;
-    const his_net1d& N = a.N;
-    ExWs w = ex_ws(a, b);
-    if (r < L.nu) {
-        const int k = r;
-        double tot = 0.0;
-        for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
-            const int l = L.u_link[p];
-            tot += w.sg[l] > 0.0 ? w.V[l] : 0.0;
-        }
-        const double av = L.limit * w.hu_[k] * L.area2;
-        w.f2[k] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
-        return;
-    }
-    const int c = r - L.nu;
-    double tot = 0.0;
-    for (int p = L.c_ptr[c]; p < L.c_ptr[c + 1]; ++p) {
-        const int l = L.c_link[p];
-        tot += w.sg[l] < 0.0 ? w.V[l] : 0.0;
-    }
-    const Sec s = cell_sec(N, b, c);
-    const double av = L.limit * clamp0(a.A[(int64_t)b * N.nc + c] - bank_area(s)) * N.dx[c];
-    w.f1[c] = tot > av ? av / (tot > 0.0 ? tot : 1.0) : 1.0;
-}
-
-// Exchange phase 3: net volume of unique 2-D cell r (r < nu) -> depth, momentum kept by
-// min(h_new / h, 1); of 1-D cell r - nu -> area.  Sums run over the target's links in link order.
+// Exchange phase 2: net volume of unique 2-D cell r (r < nu) -> depth, momentum kept by
+// min(h_new / h, 1); of 1-D cell r - nu -> area.  Each link's volume is scaled by its donor's limit
+// factor, formed here from phase 1's volumes; sums run over the target's links in link order.
 template <typename R2>
 __device__ __forceinline__ void ex_apply_item(const ExchangeArgs& a, const int b, const int r) {
     const his_links& L = a.L;
@@ -127,10 +123,11 @@ __device__ __forceinline__ void ex_apply_item(const ExchangeArgs& a, const int b
     ExWs w = ex_ws(a, b);
     if (r < L.nu) {
         const int k = r;
+        const double fk = ex_f2(w, L, k);
         double dV = 0.0;
         for (int p = L.u_ptr[k]; p < L.u_ptr[k + 1]; ++p) {
             const int l = L.u_link[p];
-            const double v = ex_final(w, L, l);
+            const double v = (w.sg[l] > 0.0 ? w.V[l] * fk : w.V[l] * ex_f1(w, L, L.cell1d[l])) * w.sg[l];
             w.Vf[l] = v;
             if (a.V) a.V[(int64_t)b * L.nl + l] = v;
             dV += v;
@@ -151,8 +148,12 @@ __device__ __forceinline__ void ex_apply_item(const ExchangeArgs& a, const int b
     const int c = r - L.nu;
     const int p0 = L.c_ptr[c], p1 = L.c_ptr[c + 1];
     if (p0 == p1) return;
+    const double fc = ex_f1(w, L, c);
     double add = 0.0;
-    for (int p = p0; p < p1; ++p) add += ex_final(w, L, L.c_link[p]);
+    for (int p = p0; p < p1; ++p) {
+        const int l = L.c_link[p];
+        add += (w.sg[l] > 0.0 ? w.V[l] * ex_f2(w, L, L.inv[l]) : w.V[l] * fc) * w.sg[l];
+    }
     double* A = a.A + (int64_t)b * N.nc;
     A[c] = A[c] + add / N.dx[c];
 }
