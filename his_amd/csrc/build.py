@@ -24,7 +24,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I", str(HERE)]
 UNITS = {
     "swe2d_f64.hip": ["-ffp-contract=off"],
-    "swe2d_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    # float32 bulk: contraction, native reciprocal / square root, and no NaN semantics (hardware
+    # min / max without operand canonicalisation); a NaN state still surfaces in the CFL rate and
+    # the volumes, which float64 code checks
+    "swe2d_f32.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fno-honor-nans"],
     "net1d.hip": ["-ffp-contract=off"],
     "engine.hip": ["-ffp-contract=off"],
 }
