@@ -13,8 +13,8 @@
 #include <cstring>
 #include <vector>
 
+#include "finalize_dev.hpp"
 #include "step_head.hpp"
-#include "swe2d_desc.hpp"
 
 namespace his {
 
@@ -39,103 +39,9 @@ int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
-    // per thread: its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22) and rate maximum (23);
-    // transposed through LDS so that thread k sums quantity k over the threads in a fixed order
-    constexpr int NV = 24;
-    __shared__ double tr[NT][NV + 1];
-    __shared__ double tot[NV];
-    const int m = blockIdx.x;
-    if (F.snap) {
-        if (F.snap[2 * F.Btot + m] == 0.0) return;     // the member did not step
-    } else if (F.active && !F.active[m]) {
-        return;
-    }
-    if (F.only && !F.only[m]) return;
-    int gi = 0;
-    while (gi + 1 < F.ngroups && m >= F.member0[gi + 1]) ++gi;
-    const int b = m - F.member0[gi];
-    const int per = F.per[gi];
-    const int64_t t0 = F.tile0[gi] + (int64_t)b * per;
-    double v[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = 0.0;
-    for (int t = threadIdx.x; t < per; t += NT) {
-        const double* p = F.part + (t0 + t) * NP2;
-        if (!F.stage_only)
-#pragma unroll
-            for (int k = 0; k <= P_NOUT; ++k) v[k] += p[k];
-#pragma unroll
-        for (int k = 0; k < P_USED - 1; ++k) v[10 + k] += p[NP + k];
-        v[23] = p[NP + P_RATE] > v[23] ? p[NP + P_RATE] : v[23];
-    }
-#pragma unroll
-    for (int k = 0; k < NV; ++k) tr[threadIdx.x][k] = v[k];
-    __syncthreads();
-    if (threadIdx.x < NV) {
-        const int k = threadIdx.x;
-        double a = 0.0;
-        if (k == 23)
-            for (int w = 0; w < NT; ++w) a = tr[w][k] > a ? tr[w][k] : a;
-        else
-            for (int w = 0; w < NT; ++w) a += tr[w][k];
-        tot[k] = a;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double s1[P_NOUT + 1], s2[P_USED - 1];
-    for (int k = 0; k <= P_NOUT; ++k) s1[k] = tot[k];
-    for (int k = 0; k < P_USED - 1; ++k) s2[k] = tot[10 + k];
-    const double rmax = tot[23];
-    const double area = F.area[gi];
-    if (F.rate) F.rate[m] = rmax;
-    if (F.volume) {
-        F.volume[m] = s2[P_VOLH] * area;
-        F.volume[F.Btot + m] = s2[P_VOLS] * area;
-    }
-    if (F.stage_only) return;
-    const double dt = F.snap ? F.snap[m] : F.dt[m], dx = F.dx[gi], dy = F.dy[gi];
-    const double q2i = (s2[P_WIN] + s2[P_EIN]) * dy + (s2[P_SIN] + s2[P_NIN]) * dx;
-    const double q2o = (s2[P_WOUT] + s2[P_EOUT]) * dy + (s2[P_SOUT] + s2[P_NOUT]) * dx;
-    double qin, qout, clip;
-    if (F.order[gi] == 2) {
-        const double q1i = (s1[P_WIN] + s1[P_EIN]) * dy + (s1[P_SIN] + s1[P_NIN]) * dx;
-        const double q1o = (s1[P_WOUT] + s1[P_EOUT]) * dy + (s1[P_SOUT] + s1[P_NOUT]) * dx;
-        qin = 0.5 * (q1i + q2i);
-        qout = 0.5 * (q1o + q2o);
-        clip = s1[P_CLIP] * area + s2[P_CLIP] * area;
-    } else {
-        qin = q2i;
-        qout = q2o;
-        clip = s2[P_CLIP] * area;
-    }
-    const double rain = F.snap ? F.snap[F.Btot + m] : F.pm[m];
-    double rain_v;
-    if (F.weighted[gi]) rain_v = s2[P_RAINW] * area;
-    else {
-        const double rd = F.rain_f32[gi] ? (double)((float)rain * (float)dt) : rain * dt;
-        rain_v = rd * (F.cells[gi] * area);
-    }
-    const double inf_v = s2[P_INF] * area;
-    const double bin = qin * dt, bout = qout * dt;
-    if (F.vols) {
-        F.vols[m] = bin;
-        F.vols[F.Btot + m] = bout;
-        F.vols[2 * F.Btot + m] = rain_v;
-        F.vols[3 * F.Btot + m] = inf_v;
-        F.vols[4 * F.Btot + m] = clip;
-    }
-    if (F.t_out && F.advance[gi]) {
-        F.t_out[m] = F.t_step[m] + dt;
-        F.steps[m] += 1;
-    }
-    if (F.acc) {
-        F.acc[0 * F.Btot + m] += rain_v;
-        F.acc[1 * F.Btot + m] += inf_v;
-        F.acc[2 * F.Btot + m] += bin;
-        F.acc[3 * F.Btot + m] += bout;
-        F.acc[6 * F.Btot + m] += clip;
-    }
-    if (F.snap) F.snap[2 * F.Btot + m] = 0.0;
+    __shared__ double tr[NT][9];
+    __shared__ double tot[24];
+    finalize_member<NT>(F, blockIdx.x, tr, tot);
 }
 
 int launch_finalize(const FinalizeArgs& f, hipStream_t s) {
@@ -438,9 +344,10 @@ struct his_engine {
     int32_t* active;
     double *pm, *rate2d, *rate1d, *volume, *t_step, *snap;
     int32_t* wet1d;
-    hipStream_t cap, side, fin;
-    hipEvent_t ev_fork, ev_join, ev_s2, ev_fin;
-    int async_fin;            // ledger reduction on its own stream, beside the next step's head
+    hipStream_t cap, side;
+    hipEvent_t ev_fork, ev_join;
+    int async_fin;            // ledger reduction deferred into the next step's first exchange launch
+    FinalizeArgs* fin_dev;    // device copy of F for that deferred reduction
     int fuse_head;            // dt and forcing evaluated inside the first exchange phase
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
@@ -449,8 +356,24 @@ struct his_engine {
     int graphs_ready;
 };
 
-// fin_pending: a ledger reduction launched on e->fin has not yet been waited for on s
-static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, bool& fin_pending) {
+// Whether a step's ledger reduction is still owed.  With async_fin it is not launched after the
+// step but carried by the next step's first exchange launch (extra blocks reading the tile
+// partials and the stage-2 snapshot), so the step chain needs no other stream; a run or a
+// captured graph ends with a standalone reduction of its last step.
+struct FinChain {
+    bool fin_due = false;
+};
+
+static int engine_close_chain(his_engine* e, hipStream_t s, FinChain& c) {
+    if (!c.fin_due) return 0;
+    c.fin_due = false;
+    return launch_finalize(e->F, s);
+}
+
+// One coupled step on s: exchange (with the previous step's ledger reduction) -> 2-D stages, the
+// 1-D network on `side` (forked after the exchange) -> join.  In a captured graph the 2-D stages
+// are captured first, so they are the exchange's first dependent and keep its hardware queue.
+static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, FinChain& chain) {
     int rc;
     if (!e->fuse_head) {
         hipLaunchKernelGGL(k_dt, dim3(e->n_dtg), dim3(e->n_dtg == 1 ? 256 : 64), 0, s, e->D);
@@ -486,32 +409,28 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, b
         X.dtg_of = e->dtg_of;
         X.rate_reset = e->rate2d;
     }
+    if (chain.fin_due) {                       // the previous step's ledger, beside the first phase
+        X.fin = e->fin_dev;
+        X.fin_members = e->Btot;
+        chain.fin_due = false;
+    }
     if ((rc = launch_exchange_groups(X, s))) return rc;
-    // every group's 1-D network in one launch, on a side stream beside the 2-D stages
     const bool any_net = W.ngroups > 0;
-    hipStream_t s1 = any_net && side ? side : s;
-    if (any_net && side) {
-        HIS_HIP(hipEventRecord(e->ev_fork, s));
-        HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
-    }
-    if ((rc = launch_swe1d_groups(W, s1))) return rc;
-    if (fin_pending) {                         // the previous step's partials must be reduced first
-        HIS_HIP(hipStreamWaitEvent(s, e->ev_fin, 0));
-        fin_pending = false;
-    }
+    const bool fork = any_net && side;
+    if (fork) HIS_HIP(hipEventRecord(e->ev_fork, s));
     rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
     if (rc) return rc;
+    // every group's 1-D network in one launch, on the side stream beside the 2-D stages
+    if (fork) HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
+    if ((rc = launch_swe1d_groups(W, fork ? side : s))) return rc;
     if (e->async_fin) {
-        // the next step's dt reads the rates the last stage folded in; only the ledger waits here
-        HIS_HIP(hipEventRecord(e->ev_s2, s));
-        HIS_HIP(hipStreamWaitEvent(e->fin, e->ev_s2, 0));
-        if ((rc = launch_finalize(e->F, e->fin))) return rc;
-        HIS_HIP(hipEventRecord(e->ev_fin, e->fin));
-        fin_pending = true;
+        // the next step's dt reads the rates the last stage folded in; the ledger reduction rides
+        // on the next step's exchange (or closes the chain)
+        chain.fin_due = true;
     } else if ((rc = launch_finalize(e->F, s))) {
         return rc;
     }
-    if (any_net && side) {
+    if (fork) {
         HIS_HIP(hipEventRecord(e->ev_join, side));
         HIS_HIP(hipStreamWaitEvent(s, e->ev_join, 0));
     }
@@ -537,7 +456,8 @@ extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
     tot += align_up((size_t)tiles * NP2 * 8);
     tot += align_up((size_t)Btot * 4) + align_up((size_t)Btot * 9 * 8) + 3 * align_up((size_t)Btot * 8) +
            align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * 3 * 8) +
-           align_up((size_t)Btot * sizeof(DtGroup)) + align_up(sizeof(DtArgs)) + align_up((size_t)Btot * 4);
+           align_up((size_t)Btot * sizeof(DtGroup)) + align_up(sizeof(DtArgs)) + align_up(sizeof(FinalizeArgs)) +
+           align_up((size_t)Btot * 4);
     return tot;
 }
 
@@ -630,6 +550,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     e->snap = (double*)p; p += align_up((size_t)Btot * 3 * 8);
     DtGroup* dtg_dev = (DtGroup*)p; p += align_up((size_t)Btot * sizeof(DtGroup));
     e->head_dev = (DtArgs*)p; p += align_up(sizeof(DtArgs));
+    e->fin_dev = (FinalizeArgs*)p; p += align_up(sizeof(FinalizeArgs));
     e->dtg_of = (int32_t*)p;
     // launches; with a network in every group the 1-D kernel advances t, so the ledger reduction is
     // off the step's critical path and the last stage publishes the CFL rates itself
@@ -691,6 +612,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
         for (int m = dtg[k].m0; m < dtg[k].m1; ++m) dtg_of[m] = k;
     he = hipMemcpyAsync(e->head_dev, &e->D, sizeof(DtArgs), hipMemcpyHostToDevice, s);
     if (he == hipSuccess) he = hipMemcpyAsync(e->dtg_of, dtg_of.data(), (size_t)Btot * 4, hipMemcpyHostToDevice, s);
+    if (he == hipSuccess) he = hipMemcpyAsync(e->fin_dev, &e->F, sizeof(FinalizeArgs), hipMemcpyHostToDevice, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(step head)"); }
     he = hipMemsetAsync(e->snap, 0, (size_t)Btot * 3 * 8, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemsetAsync(ledger snapshot)"); }
@@ -698,11 +620,8 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipStreamSynchronize"); }
     if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&e->fin, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_s2, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->ev_fin, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming) != hipSuccess) {
         delete e;
         set_error("his_engine_create: stream/event creation failed");
         return HIS_E_HIP;
@@ -752,12 +671,9 @@ static int engine_build_graphs(his_engine* e) {
         hipGraph_t g;
         HIS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));
         int rc = 0;
-        bool pend = false;
-        for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side, pend);
-        if (!rc && pend) {
-            hipError_t we = hipStreamWaitEvent(e->cap, e->ev_fin, 0);
-            if (we != hipSuccess) rc = hip_fail(we, "hipStreamWaitEvent");
-        }
+        FinChain chain;
+        for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side, chain);
+        if (!rc) rc = engine_close_chain(e, e->cap, chain);
         hipError_t he = hipStreamEndCapture(e->cap, &g);
         if (rc) return rc;
         if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture");
@@ -773,13 +689,12 @@ extern "C" int his_engine_step(his_engine* e, int32_t n_steps, void* stream) {
     if (!e || n_steps < 0) { set_error("his_engine_step: bad argument"); return HIS_E_ARG; }
     hipStream_t s = (hipStream_t)stream;
     if (!e->d.use_graph) {
-        bool pend = false;
+        FinChain chain;
         for (int i = 0; i < n_steps; ++i) {
-            int rc = engine_enqueue_step(e, s, e->side, pend);
+            int rc = engine_enqueue_step(e, s, e->side, chain);
             if (rc) return rc;
         }
-        if (pend) HIS_HIP(hipStreamWaitEvent(s, e->ev_fin, 0));
-        return 0;
+        return engine_close_chain(e, s, chain);
     }
     if (!e->graphs_ready) {
         int rc = engine_build_graphs(e);
@@ -809,11 +724,8 @@ extern "C" int his_engine_destroy(his_engine* e) {
     }
     (void)hipStreamDestroy(e->cap);
     (void)hipStreamDestroy(e->side);
-    (void)hipStreamDestroy(e->fin);
     (void)hipEventDestroy(e->ev_fork);
     (void)hipEventDestroy(e->ev_join);
-    (void)hipEventDestroy(e->ev_s2);
-    (void)hipEventDestroy(e->ev_fin);
     delete e;
     return 0;
 }

@@ -9,6 +9,7 @@
 // SWE1D._step_core  hydrointel/solver/swe1d.py:320-368 (+ _fill_ghosts 265-294,
 //                   Sections 50-84, hll_1d 87-107, max_dt 253-262)
 // exchange          hydrointel/solver/coupling.py:73-124
+#include "finalize_dev.hpp"
 #include "step_head.hpp"
 
 namespace his {
@@ -356,11 +357,21 @@ template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     __shared__ double red[4][3];
     __shared__ HeadState hs[2];
+    __shared__ double ftr[256][9];
+    __shared__ double ftot[24];
     const int64_t total = G.it_links[G.ngroups];
+    // the previous step's ledger reduction takes the first blocks, so it is dispatched first and
+    // overlaps the link blocks
+    const int nfin = G.fin ? G.fin_members : 0;
+    if ((int)blockIdx.x < nfin) {
+        finalize_member<256>(*G.fin, (int)blockIdx.x, ftr, ftot);
+        return;
+    }
+    const int64_t blk = (int64_t)blockIdx.x - nfin;
     int mA = -1, mB = -1;
     if (G.head) {
         // a block spans at most two members (nl + nu >= 256): evaluate their groups' dt here
-        const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+        const int64_t i0 = blk * blockDim.x;
         const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
         bool firstA, firstB;
         mA = ex_link_member(G, i0, firstA);
@@ -378,7 +389,7 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
             if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(*G.head, gim, mm, threadIdx.x - 64, h.dt);
         }
     }
-    int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t it = blk * (int64_t)blockDim.x + threadIdx.x;
     if (it >= total) return;
     const int gi = ex_group(G, G.it_links, it);
     const ExchangeArgs& a = G.a[gi];
@@ -443,7 +454,9 @@ int launch_exchange_groups(ExGroups& G, hipStream_t s) {
         G.it_cells[gi + 1] = G.it_cells[gi] + (int64_t)a.N.B * (a.L.nu + a.N.nc);
     }
     auto blocks = [](int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); };
-    const dim3 g1 = blocks(G.it_links[G.ngroups]), g2 = blocks(G.it_cells[G.ngroups]);
+    dim3 g1 = blocks(G.it_links[G.ngroups]);
+    const dim3 g2 = blocks(G.it_cells[G.ngroups]);
+    if (G.fin) g1.x += (unsigned)G.fin_members;
     if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, G);
     else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());

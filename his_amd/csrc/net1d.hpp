@@ -53,6 +53,7 @@ struct ExchangeArgs {
 int launch_exchange(const ExchangeArgs& a, hipStream_t s);
 
 struct DtArgs;
+struct FinalizeArgs;
 
 // Several groups' exchanges in one launch per phase (every group shares the 2-D dtype).
 struct ExGroups {
@@ -65,6 +66,10 @@ struct ExGroups {
     const DtArgs* head;
     const int32_t* dtg_of;
     double* rate_reset;      // (Btot) or NULL: cleared in the last phase for every member that steps
+    // optional: the previous step's ledger reduction (device copy of its arguments), one extra
+    // 256-thread block per member after the link blocks of the first phase
+    const FinalizeArgs* fin;
+    int fin_members;
 };
 int launch_exchange_groups(ExGroups& g, hipStream_t s);
 
