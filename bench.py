@@ -197,16 +197,21 @@ def traffic_from_profile():
     return None
 
 
-def kernel_only(reps: int):
+def kernel_only(reps: int, shape: int | None = None):
     """The 2-D step of the headline workload alone, then a 256 MiB copy of known size: the run the
-    FETCH_SIZE / WRITE_SIZE counter passes profile (the copy calibrates the counters' units)."""
+    FETCH_SIZE / WRITE_SIZE counter passes profile (the copy calibrates the counters' units).
+    ``shape`` fixes the float32 kernel shape (default: the autotuner's choice)."""
     import torch
     from his_amd import tuning
     from his_amd import workloads as W
     from his_amd.engine import build_from_workload
     wl = W.precision_form()
     eng, _ = build_from_workload(wl, device=torch.device("cuda"))
-    shape = tuning.tune(eng.two)
+    if shape is None:
+        shape = tuning.tune(eng.two)
+    else:
+        from his_amd import _lib
+        _lib.set_tile_shape(torch.float32, shape)
     log(f"[kernel-only] shape {shape} ({tuning.describe(eng.two.dtype, shape)}), {reps} steps")
     de = eng._device()          # advance the storm 2000 coupled steps so the step is timed on a wet state
     de.t.fill_(0.0)
@@ -236,14 +241,15 @@ def main():
     ap.add_argument("--serve", type=int, default=120, help="requests of the serving sample (0 = skip)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="run only the 2-D step (and a calibration copy) for counter profiling; prints no JSON")
+    ap.add_argument("--shape", type=int, default=None, help="--kernel-only: fixed float32 kernel shape index")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the storms, members and requests of this engine are independent and "
-              "would shard across GPUs with no exchange; multi-GPU runs are not part of this build.",
+              "would shard across GPUs with no exchange; runs on several GPUs are not supported.",
               file=sys.stderr)
         sys.exit(2)
     if args.kernel_only:
-        kernel_only(args.steps)
+        kernel_only(args.steps, args.shape)
         return
     import torch
     from his_amd import workloads as W

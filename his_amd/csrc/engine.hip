@@ -39,9 +39,9 @@ int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
-    __shared__ double tr[NT][9];
+    __shared__ double red[NT / 64][24];
     __shared__ double tot[24];
-    finalize_member<NT>(F, blockIdx.x, tr, tot);
+    finalize_member<NT>(F, blockIdx.x, red, tot);
 }
 
 int launch_finalize(const FinalizeArgs& f, hipStream_t s) {
@@ -349,6 +349,7 @@ struct his_engine {
     int async_fin;            // ledger reduction deferred into the next step's first exchange launch
     FinalizeArgs* fin_dev;    // device copy of F for that deferred reduction
     int fuse_head;            // dt and forcing evaluated inside the first exchange phase
+    int one_first;            // capture the 1-D network before the 2-D stages (the 1-D branch is longer)
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
@@ -371,8 +372,9 @@ static int engine_close_chain(his_engine* e, hipStream_t s, FinChain& c) {
 }
 
 // One coupled step on s: exchange (with the previous step's ledger reduction) -> 2-D stages, the
-// 1-D network on `side` (forked after the exchange) -> join.  In a captured graph the 2-D stages
-// are captured first, so they are the exchange's first dependent and keep its hardware queue.
+// 1-D network on `side` (forked after the exchange) -> join.  In a captured graph the exchange's
+// first-captured dependent keeps its hardware queue and the other branch pays a cross-queue start,
+// so the branch expected to be longer (one_first) is captured first.
 static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, FinChain& chain) {
     int rc;
     if (!e->fuse_head) {
@@ -418,11 +420,15 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
     const bool any_net = W.ngroups > 0;
     const bool fork = any_net && side;
     if (fork) HIS_HIP(hipEventRecord(e->ev_fork, s));
+    // every group's 1-D network in one launch, on the side stream beside the 2-D stages
+    auto one_d = [&]() -> int {
+        if (fork) HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
+        return launch_swe1d_groups(W, fork ? side : s);
+    };
+    if (e->one_first && (rc = one_d())) return rc;
     rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
     if (rc) return rc;
-    // every group's 1-D network in one launch, on the side stream beside the 2-D stages
-    if (fork) HIS_HIP(hipStreamWaitEvent(side, e->ev_fork, 0));
-    if ((rc = launch_swe1d_groups(W, fork ? side : s))) return rc;
+    if (!e->one_first && (rc = one_d())) return rc;
     if (e->async_fin) {
         // the next step's dt reads the rates the last stage folded in; the ledger reduction rides
         // on the next step's exchange (or closes the chain)
@@ -594,6 +600,15 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     for (int gi = 0; gi < d->ngroups; ++gi)
         if (e->has_net[gi] && e->links[gi].nl + e->links[gi].nu < 256) e->fuse_head = 0;
     D.rate2d_reset = e->async_fin && !e->fuse_head ? e->rate2d : nullptr;
+    // which branch of a step is longer: the 1-D network takes ~70-140 us per step whatever the batch
+    // (one workgroup per member, all members at once); the 2-D stages scale with the member-cells
+    // (~23 G/s in float32, ~4 G/s in float64 on this part)
+    {
+        double cells = 0.0;
+        for (int gi = 0; gi < d->ngroups; ++gi) cells += (double)d->group[gi].grid.B * d->group[gi].grid.ny * d->group[gi].grid.nx;
+        const double est_2d_us = cells / (dtype == HIS_F64 ? 4.0e3 : 2.3e4);
+        e->one_first = est_2d_us < 100.0;
+    }
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];

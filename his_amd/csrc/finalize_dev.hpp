@@ -4,8 +4,8 @@
 // launch so that the reduction needs no stream of its own.
 //
 // Each thread sums its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22) and rate maximum
-// (23); the 24 per-thread values then pass through LDS eight at a time, and thread k of a pass
-// sums its quantity over the threads 0..NT-1 in order, so the result is bit-reproducible.
+// (23); each quantity is then reduced by a fixed butterfly within every wave and the waves' results
+// are combined in wave order, so the result is bit-reproducible.
 // Volumes follow hydrointel/solver/swe2d.py:_step_core 305-371: boundary volumes from the RK
 // average of the two stages' edge fluxes (324-328), rain (331-337), infiltration and clipping.
 #pragma once
@@ -14,7 +14,7 @@
 namespace his {
 
 template <int NT>
-__device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int m, double (*tr)[9], double* tot) {
+__device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int m, double (*red)[24], double* tot) {
     constexpr int NV = 24;
     if (F.snap) {
         if (F.snap[2 * F.Btot + m] == 0.0) return;     // the member did not step
@@ -39,21 +39,19 @@ __device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int
         for (int k = 0; k < P_USED - 1; ++k) v[10 + k] += p[NP + k];
         v[23] = p[NP + P_RATE] > v[23] ? p[NP + P_RATE] : v[23];
     }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int c = 0; c < NV / 8; ++c) {
-        __syncthreads();
+    for (int k = 0; k < NV - 1; ++k) v[k] = wave_sum(v[k]);
+    v[NV - 1] = wave_max(v[NV - 1]);
+    if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) tr[threadIdx.x][k] = v[8 * c + k];
-        __syncthreads();
-        if (threadIdx.x < 8) {
-            const int k = 8 * c + threadIdx.x;
-            double a = 0.0;
-            if (k == 23)
-                for (int w = 0; w < NT; ++w) a = tr[w][threadIdx.x] > a ? tr[w][threadIdx.x] : a;
-            else
-                for (int w = 0; w < NT; ++w) a += tr[w][threadIdx.x];
-            tot[k] = a;
-        }
+        for (int k = 0; k < NV; ++k) red[wv][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int k = threadIdx.x;
+        double a = red[0][k];
+        for (int w = 1; w < NT / 64; ++w) a = k == NV - 1 ? (red[w][k] > a ? red[w][k] : a) : a + red[w][k];
+        tot[k] = a;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;

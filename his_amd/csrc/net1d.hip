@@ -357,14 +357,14 @@ template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     __shared__ double red[4][3];
     __shared__ HeadState hs[2];
-    __shared__ double ftr[256][9];
+    __shared__ double fred[4][24];
     __shared__ double ftot[24];
     const int64_t total = G.it_links[G.ngroups];
     // the previous step's ledger reduction takes the first blocks, so it is dispatched first and
     // overlaps the link blocks
     const int nfin = G.fin ? G.fin_members : 0;
     if ((int)blockIdx.x < nfin) {
-        finalize_member<256>(*G.fin, (int)blockIdx.x, ftr, ftot);
+        finalize_member<256>(*G.fin, (int)blockIdx.x, fred, ftot);
         return;
     }
     const int64_t blk = (int64_t)blockIdx.x - nfin;
