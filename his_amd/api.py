@@ -118,6 +118,13 @@ def _run(sites, forcing: Forcing, terrain, network, output_interval_s, cfg, devi
     return eng, snaps, err
 
 
+def _member(v, b: int, batched: bool):
+    """Member b's value of a ledger or mass-series entry (arrays over the batch when batched)."""
+    if batched and getattr(v, "ndim", 0):
+        return float(np.asarray(v, dtype=np.float64).reshape(-1)[b])
+    return float(v) if isinstance(v, (int, float, np.floating, np.integer)) else v
+
+
 def _result(eng, snaps, err, b, batched: bool, prov: dict, extras: dict) -> FloodResult:
     interior = torch.as_tensor(eng.one.topo.kind <= 1)
     pick = (lambda a: a[b]) if batched else (lambda a: a)
@@ -135,7 +142,12 @@ def _result(eng, snaps, err, b, batched: bool, prov: dict, extras: dict) -> Floo
         mass_balance_error=float(np.asarray(err, dtype=np.float64).reshape(-1)[b]), provenance=prov,
         extras={"run_log": eng.log, "sources": s_b, "channel_cells": np.nonzero(eng.one.topo.kind <= 1)[0],
                 "topology": eng.one.topo, "boundary_in_m3": s_b["bnd2d_in"] + s_b["bnd1d_in"],
-                "boundary_out_m3": s_b["bnd2d_out"] + s_b["bnd1d_out"], **extras})
+                "boundary_out_m3": s_b["bnd2d_out"] + s_b["bnd1d_out"],
+                # volume ledger at each output time and the mass-balance checks, this member's
+                # (hydrointel/api.py:245-247, 315-320)
+                "ledger": [{k: _member(v, b, batched) for k, v in x.ledger.items()} for x in snaps],
+                "mass_series": [{k: _member(v, b, batched) for k, v in rec.items()} for rec in eng.mb.series],
+                **extras})
 
 
 def simulate(site: SiteState, forcing: Forcing, terrain, network, output_interval_s: float | None = None,
@@ -170,7 +182,7 @@ def simulate_batch(sites: list, forcing: Forcing, terrain, network, output_inter
     eng, snaps, err = _run(sites, forcing, terrain, network, output_interval_s, cfg, device,
                            f"batch of {len(sites)}")
     prov = dict(engine="his_amd", precision=cfg.precision, device=str(eng.two.device))
-    return [_result(eng, snaps, err, b, True, prov, {"batch_index": b, "batch_size": len(sites)})
+    return [_result(eng, snaps, err, b, True, prov, {"forcing": forcing, "batch_index": b, "batch_size": len(sites)})
             for b in range(len(sites))]
 
 
