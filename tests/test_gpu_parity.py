@@ -427,6 +427,40 @@ def test_served_request_equals_the_request_alone(gpu, precision):
         assert abs(r.mass_err - alone["mass_err"]) <= 1e-12 + 1e-9 * alone["mass_err"]
 
 
+def test_a_failing_request_fails_alone(gpu):
+    """A request whose member the engine flags stops and is reported unstable; the requests
+    beside it, in the same pool and the same launches, finish with their standalone results.
+    The flag is set from the host right after admission, as the step head sets it on a
+    non-finite rate or too many 1-D substeps, so every kernel still runs normally."""
+    from his_amd import serving as S
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+
+    class Flagging(S.Server):
+        def _admit(self, pi, req, now):
+            ok = super()._admit(pi, req, now)
+            if ok and req["id"] == 1:
+                m = next(m for m, v in self.busy.items() if v[2].id == 1)
+                self.dev.error[m] = 2
+            return ok
+
+    grids = (S.GridSpec(100, 81, 80.0, "chain"),)
+    ms = W.draw_members(3, 91)
+    reqs = [dict(id=i, arrival_s=0.0, size=0, storm_s=s, member=ms[i]) for i, s in enumerate((1200.0, 1800.0, 960.0))]
+    srv = Flagging(slots=(3,), precision="fp64", device=gpu, grids=grids, chunk=16, keep_fields=True, tune=False)
+    res = {r.id: r for r in srv.serve(reqs, realtime=False, log_every_s=0)}
+    srv.close()
+    assert sorted(res) == [0, 1, 2]
+    assert res[1].status == "unstable: non-finite CFL rate" and res[1].steps == 0, (res[1].status, res[1].steps)
+    cfg = SolverConfig().replace(precision="fp64")
+    for i in (0, 2):
+        r = res[i]
+        assert r.status == "ok", (i, r.status)
+        alone = S.run_alone(grids[0], reqs[i], cfg, gpu)
+        assert r.steps == alone["steps"]
+        assert np.array_equal(r.fields["h"], alone["h"]) and np.array_equal(r.fields["A"], alone["A"])
+
+
 def test_simulate_and_simulate_batch(gpu):
     """FloodResult assembly: depth_max over the output-time snapshots, per-member mass balance,
     a batch of one routed to simulate, and the zero disbenefit of a design against itself."""
