@@ -350,6 +350,7 @@ struct his_engine {
     FinalizeArgs* fin_dev;    // device copy of F for that deferred reduction
     int fuse_head;            // dt and forcing evaluated inside the first exchange phase
     int one_first;            // capture the 1-D network before the 2-D stages (the 1-D branch is longer)
+    int ex_one;               // every 1-D cell has at most one link: the exchange is one launch
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
@@ -406,10 +407,13 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
         d.member0 = m0;
     }
+    X.one_phase = e->ex_one;
     if (e->fuse_head) {
         X.head = e->head_dev;
         X.dtg_of = e->dtg_of;
-        X.rate_reset = e->rate2d;
+        // the second phase clears the rates once every block has read them; with a single phase
+        // stage 1 of the 2-D step does it (Launch2D::rate_reset)
+        X.rate_reset = e->ex_one ? nullptr : e->rate2d;
     }
     if (chain.fin_due) {                       // the previous step's ledger, beside the first phase
         X.fin = e->fin_dev;
@@ -595,11 +599,33 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     D = DtArgs{};
     D.groups = dtg_dev; D.ngroups_dt = e->n_dtg; D.cfl = d->cfl; D.dt_min = d->dt_min; D.dt_max = d->dt_max;
     D.ctl = c; D.active = e->active; D.t_step = e->t_step; D.pm = e->pm; D.rate2d = e->rate2d; D.rate1d = e->rate1d; D.wet1d = e->wet1d;
-    // the fused head needs every member's links to span a whole block (two members per block at most)
+    // single-phase exchange: second-order engines whose 1-D cells each have at most one link (the
+    // rates it leaves to be cleared are cleared by stage 1, which a first-order step does not run)
+    e->ex_one = order == 2;
+    for (int gi = 0; gi < d->ngroups && e->ex_one; ++gi) {
+        if (!e->has_net[gi]) continue;
+        const int nc = e->net[gi].nc;
+        std::vector<int32_t> cp((size_t)nc + 1);
+        hipStream_t s0 = (hipStream_t)stream;
+        if (hipMemcpyAsync(cp.data(), e->links[gi].c_ptr, cp.size() * 4, hipMemcpyDeviceToHost, s0) != hipSuccess ||
+            hipStreamSynchronize(s0) != hipSuccess) {
+            delete e;
+            set_error("his_engine_create: reading the link table failed");
+            return HIS_E_HIP;
+        }
+        for (int c = 0; c < nc; ++c)
+            if (cp[c + 1] - cp[c] > 1) { e->ex_one = 0; break; }
+    }
+    // the fused head needs every member's first-launch items to span a whole block (two members
+    // per block at most): links and unique cells, or unique cells alone with one phase
     e->fuse_head = e->async_fin;
     for (int gi = 0; gi < d->ngroups; ++gi)
-        if (e->has_net[gi] && e->links[gi].nl + e->links[gi].nu < 256) e->fuse_head = 0;
+        if (e->has_net[gi] && (e->ex_one ? e->links[gi].nu : e->links[gi].nl + e->links[gi].nu) < 256) e->fuse_head = 0;
     D.rate2d_reset = e->async_fin && !e->fuse_head ? e->rate2d : nullptr;
+    if (e->async_fin && e->ex_one) {
+        if (dtype == HIS_F64) e->L64.rate_reset = e->rate2d;
+        else e->L32.rate_reset = e->rate2d;
+    }
     // which branch of a step is longer: the 1-D network takes ~70-140 us per step whatever the batch
     // (one workgroup per member, all members at once); the 2-D stages scale with the member-cells
     // (~23 G/s in float32, ~4 G/s in float64 on this part)

@@ -347,7 +347,7 @@ __device__ __forceinline__ int ex_link_member(const ExGroups& G, int64_t it, boo
     const int gi = ex_group(G, G.it_links, it);
     const ExchangeArgs& a = G.a[gi];
     const int64_t local = it - G.it_links[gi];
-    const int per = a.L.nl + a.L.nu;
+    const int per = G.one_phase ? a.L.nu : a.L.nl + a.L.nu;
     const int b = (int)(local / per);
     first = local == (int64_t)b * per;
     return a.member0 + b;
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     const int gi = ex_group(G, G.it_links, it);
     const ExchangeArgs& a = G.a[gi];
     it -= G.it_links[gi];
-    const int per = a.L.nl + a.L.nu;
+    const int per = G.one_phase ? a.L.nu : a.L.nl + a.L.nu;
     const int b = (int)(it / per), r = (int)(it - (int64_t)b * per);
     const int m = a.member0 + b;
     double dt;
@@ -406,7 +406,8 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
         if (a.active && !a.active[m]) return;
         dt = a.dt[m];
     }
-    ex_link_item<R2>(a, b, r, dt);
+    if (G.one_phase) ex_one_item<R2>(a, b, r, dt);
+    else ex_link_item<R2>(a, b, r, dt);
 }
 
 template <typename R2>
@@ -450,7 +451,7 @@ int launch_exchange_groups(ExGroups& G, hipStream_t s) {
     for (int gi = 0; gi < G.ngroups; ++gi) {
         const ExchangeArgs& a = G.a[gi];
         if (a.dtype2d != G.a[0].dtype2d) return HIS_E_ARG;
-        G.it_links[gi + 1] = G.it_links[gi] + (int64_t)a.N.B * (a.L.nl + a.L.nu);
+        G.it_links[gi + 1] = G.it_links[gi] + (int64_t)a.N.B * (G.one_phase ? a.L.nu : a.L.nl + a.L.nu);
         G.it_cells[gi + 1] = G.it_cells[gi] + (int64_t)a.N.B * (a.L.nu + a.N.nc);
     }
     auto blocks = [](int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); };
@@ -460,9 +461,11 @@ int launch_exchange_groups(ExGroups& G, hipStream_t s) {
     if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_links<double>, g1, dim3(256), 0, s, G);
     else hipLaunchKernelGGL(k_ex_links<float>, g1, dim3(256), 0, s, G);
     HIS_HIP(hipGetLastError());
-    if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, G);
-    else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, G);
-    HIS_HIP(hipGetLastError());
+    if (!G.one_phase) {
+        if (G.a[0].dtype2d == HIS_F64) hipLaunchKernelGGL(k_ex_apply<double>, g2, dim3(256), 0, s, G);
+        else hipLaunchKernelGGL(k_ex_apply<float>, g2, dim3(256), 0, s, G);
+        HIS_HIP(hipGetLastError());
+    }
     for (int gi = 0; gi < G.ngroups; ++gi) {
         const ExchangeArgs& a = G.a[gi];
         if (!a.acc) continue;

@@ -70,6 +70,9 @@ struct ExGroups {
     // 256-thread block per member after the link blocks of the first phase
     const FinalizeArgs* fin;
     int fin_members;
+    // every group's 1-D cells have at most one link each: the first launch does the whole exchange
+    // (one item per unique 2-D cell) and no second phase runs
+    int one_phase;
 };
 int launch_exchange_groups(ExGroups& g, hipStream_t s);
 

@@ -113,6 +113,68 @@ __device__ __forceinline__ void ex_link_item(const ExchangeArgs& a, const int b,
     if (L.c_link[L.c_ptr[c1]] == l) w.av1[c1] = L.limit * clamp0(A1 - bank_area(s)) * N.dx[c1];
 }
 
+// Single-phase exchange of unique 2-D cell k of member b, for link tables in which every 1-D cell
+// has at most one link (one link per interior 1-D cell, as hydrointel/solver/coupling.py
+// build_links makes them): the cell's links, both donor limits and both applications are this
+// item's alone, so one thread does what the two phases do, with the same operations in the same
+// order (identical results).  The workspace receives V, sign, 1-D donor volume and final V as the
+// two-phase form leaves them.
+template <typename R2>
+__device__ __forceinline__ void ex_one_item(const ExchangeArgs& a, const int b, const int k, const double dt) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    ExWs w = ex_ws(a, b);
+    R2* h = reinterpret_cast<R2*>(a.h);
+    const int64_t q = (int64_t)b * a.plane + L.uniq[k];
+    const double h2 = (double)h[q];
+    const int p0 = L.u_ptr[k], p1 = L.u_ptr[k + 1];
+    double* A = a.A + (int64_t)b * N.nc;
+    // phase 1: each link's free-weir volume capped by the level-equalising volume, its sign, and
+    // its 1-D cell's available donor volume
+    for (int p = p0; p < p1; ++p) {
+        const int l = L.u_link[p];
+        const int c1 = L.cell1d[l];
+        const Sec s = cell_sec(N, b, c1);
+        const double eta2 = L.z2[l] + h2;
+        const double A1 = A[c1];
+        const double y1 = sec_depth(s, A1);
+        const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
+        const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
+        const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
+        const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
+        const double qw = L.weir_cw * L.length[l] * (d_up * sqrt(d_up)) * pow(clamp0(1.0 - ratio * sqrt(ratio)), 0.385);
+        w.sg[l] = eta2 > eta1 ? 1.0 : -1.0;
+        const double v = qw * dt;
+        const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * N.dx[c1];
+        const double v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
+        w.V[l] = tmin(v, L.relax * v_eq);
+        w.av1[c1] = L.limit * clamp0(A1 - bank_area(s)) * N.dx[c1];
+    }
+    w.hu_[k] = h2;
+    // phase 2: limit factors, final volumes, and the application to the 2-D cell and the 1-D cells
+    const double fk = ex_f2(w, L, k);
+    double dV = 0.0;
+    for (int p = p0; p < p1; ++p) {
+        const int l = L.u_link[p];
+        const int c1 = L.cell1d[l];
+        const double v = (w.sg[l] > 0.0 ? w.V[l] * fk : w.V[l] * ex_f1(w, L, c1)) * w.sg[l];
+        w.Vf[l] = v;
+        if (a.V) a.V[(int64_t)b * L.nl + l] = v;
+        dV += v;
+        double add = 0.0;
+        add += v;
+        A[c1] = A[c1] + add / N.dx[c1];
+    }
+    const double hn = h2 - dV / L.area2;
+    const double keep = h2 > 0.0 ? tmin(hn / (h2 > 0.0 ? h2 : 1.0), 1.0) : 0.0;
+    R2* hu = reinterpret_cast<R2*>(a.hu);
+    R2* hv = reinterpret_cast<R2*>(a.hv);
+    const R2 kp = (R2)keep;
+    h[q] = (R2)hn;
+    hu[q] = hu[q] * kp;
+    hv[q] = hv[q] * kp;
+}
+
 // Exchange phase 2: net volume of unique 2-D cell r (r < nu) -> depth, momentum kept by
 // min(h_new / h, 1); of 1-D cell r - nu -> area.  Each link's volume is scaled by its donor's limit
 // factor, formed here from phase 1's volumes; sums run over the target's links in link order.

@@ -46,6 +46,9 @@ struct Launch2D {
     // (3, Btot) or NULL: the last stage's first tile of each member records that member's dt, rain
     // rate and a 1 (stepped) for a ledger reduction that runs while the next step's dt is chosen
     double *snap;
+    // (Btot) or NULL: stage 1's first tile of each stepping member clears rate_reset[m] (the rates
+    // the last stage folds into with atomic max), after every reader of the previous rates has run
+    double *rate_reset;
 };
 
 // Kernel shapes compiled for each precision; the index into this table is the tuning knob.
