@@ -113,7 +113,7 @@ def test_swe1d_step_matches_reference(golden, gpu, name):
         assert_rel(got, d["vols"][s], RTOL, 1e-9, f"step {s} volumes")
 
 
-def coupled_case(device, dtype_precision="fp64", B=2):
+def coupled_case(device, dtype_precision="fp64", B=2, order=2):
     """The golden coupled workload: 100x81 @ 80 m, 20-reach chain, two members."""
     from his_amd import workloads as W
     from his_amd.config import SolverConfig
@@ -122,7 +122,7 @@ def coupled_case(device, dtype_precision="fp64", B=2):
     wl.members = [wl.members[0], W.Member(manning=0.05, retention_m=0.008, f0_mmh=40.0, gamma=1.2,
                                           rain_peak_mmh=110.0, tide_phase=1.3, inflow_peak=220.0)][:B]
     wl.precision = dtype_precision
-    eng, fo = build_from_workload(wl, SolverConfig(), device, batched=True)
+    eng, fo = build_from_workload(wl, SolverConfig(spatial_order=order), device, batched=True)
     return eng, fo
 
 
@@ -210,12 +210,15 @@ def test_device_engine_survives_snapshots_and_checks():
     assert_rel(_f64(eng_b.one.Q), _f64(eng_a.one.Q), 1e-12, 1e-12, "Q")
 
 
-def test_device_engine_matches_oracle(gpu):
-    """Device engine vs the float64 CPU oracle running the same coupled order, 30 steps."""
+@pytest.mark.parametrize("order", [2, 1])
+def test_device_engine_matches_oracle(gpu, order):
+    """Device engine vs the float64 CPU oracle running the same coupled order, 30 steps.  Second
+    order takes the single-phase exchange (one link per 1-D cell); first order keeps the two
+    phases, since its one 2-D stage cannot clear the CFL rates the exchange reads."""
     from oracle.oracle import Oracle2D, OracleCoupled, OracleNet, oracle_links
     from his_amd import workloads as W
     from his_amd.domain.channels import apply_gamma
-    eng, fo = coupled_case(gpu)
+    eng, fo = coupled_case(gpu, order=order)
     wl = W.plain_cpu()
     members = [wl.members[0], W.Member(manning=0.05, retention_m=0.008, f0_mmh=40.0, gamma=1.2,
                                        rain_peak_mmh=110.0, tide_phase=1.3, inflow_peak=220.0)]
@@ -223,7 +226,8 @@ def test_device_engine_matches_oracle(gpu):
     n, S, f0, fc, k = wl.fields()
     ter = wl.terrain
     two = Oracle2D(ter.z, n, ter.dx, dict(west="transmissive", east="stage", south="transmissive",
-                                          north="transmissive"), B=2, storage=S, horton=(f0, fc, k, 0.0))
+                                          north="transmissive"), B=2, storage=S, horton=(f0, fc, k, 0.0),
+                   order=order)
     one = OracleNet([apply_gamma(wl.network, m.gamma) for m in members])
     orc = OracleCoupled(two, one, oracle_links(one, ter.z, ter.dx), dict(east=fo.tide), fo.rain, fo.inflow,
                         fo.tide)
