@@ -320,10 +320,11 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
     __shared__ double red[4][3];
     __shared__ HeadState hs;
     head_group_dt(a, G, red, hs);
-    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) head_member_ctl(a, m, hs, a.rate2d_reset);
+    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) head_member_ctl(a, m, hs, a.rate2d_reset, a.ctl.t[m]);
     if (!hs.go) return;
     const int nm = G.m1 - G.m0;
-    for (int k = threadIdx.x; k < nm * 9; k += blockDim.x) head_member_pm(a, G.gi, G.m0 + k / 9, k % 9, hs.dt);
+    for (int k = threadIdx.x; k < nm * 9; k += blockDim.x)
+        head_member_pm(a, G.gi, G.m0 + k / 9, k % 9, hs.dt, a.ctl.t[G.m0 + k / 9]);
 }
 
 }  // namespace his

@@ -12,6 +12,8 @@
 #include "finalize_dev.hpp"
 #include "step_head.hpp"
 
+#include <type_traits>
+
 namespace his {
 
 template <int NT, int NV>
@@ -299,18 +301,47 @@ __global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const doub
 
 size_t swe1d_ws_doubles(const his_net1d& n) { return (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf); }
 
+// Workgroup size for a member's network: one thread per cell and per face where the network
+// fits in 1024, in whole waves. Idle waves would still meet every barrier of every substep and
+// hold wave slots the 2-D stages running beside the network could use.
+static int swe1d_threads(const his_net1d& n) {
+    const int w = n.nc > n.nf ? n.nc : n.nf;
+    int nt = 64;
+    while (nt < w && nt < 1024) nt *= 2;
+    return nt;
+}
+
+template <typename F>
+static int with_threads(int nt, F&& launch) {
+    switch (nt) {
+    case 64: return launch(std::integral_constant<int, 64>{});
+    case 128: return launch(std::integral_constant<int, 128>{});
+    case 256: return launch(std::integral_constant<int, 256>{});
+    case 512: return launch(std::integral_constant<int, 512>{});
+    default: return launch(std::integral_constant<int, 1024>{});
+    }
+}
+
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
                  hipStream_t s) {
-    hipLaunchKernelGGL(k_swe1d<1024>, dim3(nblocks), dim3(1024), 0, s, n, A, Q, d, ws);
-    HIS_HIP(hipGetLastError());
-    return 0;
+    return with_threads(swe1d_threads(n), [&](auto nt) {
+        constexpr int NT = decltype(nt)::value;
+        hipLaunchKernelGGL(k_swe1d<NT>, dim3(nblocks), dim3(NT), 0, s, n, A, Q, d, ws);
+        HIS_HIP(hipGetLastError());
+        return 0;
+    });
 }
 
 int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s) {
     if (g.ngroups < 1) return 0;
-    hipLaunchKernelGGL(k_swe1d_groups<1024>, dim3(g.blk0[g.ngroups]), dim3(1024), 0, s, g);
-    HIS_HIP(hipGetLastError());
-    return 0;
+    int nt = 64;
+    for (int gi = 0; gi < g.ngroups; ++gi) nt = swe1d_threads(g.N[gi]) > nt ? swe1d_threads(g.N[gi]) : nt;
+    return with_threads(nt, [&](auto t) {
+        constexpr int NT = decltype(t)::value;
+        hipLaunchKernelGGL(k_swe1d_groups<NT>, dim3(g.blk0[g.ngroups]), dim3(NT), 0, s, g);
+        HIS_HIP(hipGetLastError());
+        return 0;
+    });
 }
 
 int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
@@ -353,6 +384,57 @@ __device__ __forceinline__ int ex_link_member(const ExGroups& G, int64_t it, boo
     return a.member0 + b;
 }
 
+// The step head fused with the single-phase exchange (one item per unique 2-D cell).  Both have a
+// chain of dependent loads: the head's from the block's members to their dt group and its rates and
+// clock, the item's from the cell to its link, its 1-D cell and that cell's section and area.  The
+// item's dt-free part (ex_one_pre) is written between the head's loads and its reduction, so the two
+// chains are in flight together; only ex_one_post waits for dt.
+template <typename R2>
+__device__ __forceinline__ void ex_head_one(const ExGroups& G, const int64_t blk, double (*red)[3], HeadState* hs) {
+    const DtArgs& D = *G.head;
+    const int64_t total = G.it_links[G.ngroups];
+    const int64_t i0 = blk * blockDim.x;
+    const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
+    bool firstA, firstB;
+    const int mA = ex_link_member(G, i0, firstA);
+    const int mB = ex_link_member(G, i1, firstB);
+    const bool two = mB != mA;
+    // this thread's item; threads past the last item load the last item's data and write nothing
+    const bool mine = i0 + (int64_t)threadIdx.x < total;
+    const int64_t it = mine ? i0 + (int64_t)threadIdx.x : total - 1;
+    const int gi = ex_group(G, G.it_links, it);
+    const ExchangeArgs& a = G.a[gi];
+    const int64_t loc = it - G.it_links[gi];
+    const int b = (int)(loc / a.L.nu), k = (int)(loc - (int64_t)b * a.L.nu);
+    const bool single = a.L.u_ptr[k + 1] - a.L.u_ptr[k] == 1;
+    const DtGroup GA = D.groups[G.dtg_of[mA]];
+    const DtGroup GB = D.groups[G.dtg_of[mB]];
+    HeadLoads vA, vB;
+    head_loads(D, GA, vA);
+    if (two) head_loads(D, GB, vB);
+    // the clocks of the members whose control entries and forcing this block writes
+    const double tA = D.ctl.t[mA], tB = D.ctl.t[mB];
+    ExPre<R2> e;
+    ex_one_pre<R2>(a, b, k, e);        // a cell with several links redoes its item after the head
+    head_reduce(D, GA, vA, red, hs[0]);
+    if (two) head_reduce(D, GB, vB, red, hs[1]);
+    for (int j = 0; j < 2; ++j) {
+        const int mm = j == 0 ? mA : mB;
+        const bool writes = j == 0 ? firstA : two;
+        if (!writes) continue;
+        const HeadState& h = hs[j];
+        const int gim = (j == 0 ? GA : GB).gi;
+        const double tm = j == 0 ? tA : tB;
+        if (threadIdx.x == 0) head_member_ctl(D, mm, h, nullptr, tm);
+        if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(D, gim, mm, threadIdx.x - 64, h.dt, tm);
+    }
+    if (!mine) return;
+    const HeadState& h = a.member0 + b == mA ? hs[0] : hs[1];
+    if (!h.go) return;
+    if (single) ex_one_post<R2>(a, b, k, e, h.dt);
+    else ex_one_item<R2>(a, b, k, h.dt);
+}
+
 template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     __shared__ double red[4][3];
@@ -368,6 +450,10 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
         return;
     }
     const int64_t blk = (int64_t)blockIdx.x - nfin;
+    if (G.head && G.one_phase) {
+        ex_head_one<R2>(G, blk, red, hs);
+        return;
+    }
     int mA = -1, mB = -1;
     if (G.head) {
         // a block spans at most two members (nl + nu >= 256): evaluate their groups' dt here
@@ -385,8 +471,9 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
             if (!mine) continue;
             const HeadState& h = hs[j];
             const int gim = G.head->groups[G.dtg_of[mm]].gi;
-            if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr);
-            if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(*G.head, gim, mm, threadIdx.x - 64, h.dt);
+            const double tm = G.head->ctl.t[mm];
+            if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr, tm);
+            if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(*G.head, gim, mm, threadIdx.x - 64, h.dt, tm);
         }
     }
     int64_t it = blk * (int64_t)blockDim.x + threadIdx.x;

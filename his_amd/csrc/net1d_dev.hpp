@@ -175,6 +175,72 @@ __device__ __forceinline__ void ex_one_item(const ExchangeArgs& a, const int b, 
     hv[q] = hv[q] * kp;
 }
 
+// ex_one_item for a unique 2-D cell with exactly one link, split where dt enters: ex_one_pre does
+// the loads and the arithmetic that do not depend on dt (the weir law, the level-equalising volume,
+// the sign, the 1-D cell's donor volume), ex_one_post the rest.  Every operation is ex_one_item's,
+// in its order.  Nothing else in the launch writes the cell's h, hu, hv or its 1-D cell's area
+// (each belongs to this item alone), so reading them before dt is known reads the same values.
+template <typename R2>
+struct ExPre {
+    int64_t q;
+    int l, c1;
+    double h2, A1, dx1, qw, v_eq, sg, av1;
+    R2 hu, hv;
+};
+
+template <typename R2>
+__device__ __forceinline__ void ex_one_pre(const ExchangeArgs& a, const int b, const int k, ExPre<R2>& e) {
+    const his_links& L = a.L;
+    const his_net1d& N = a.N;
+    e.q = (int64_t)b * a.plane + L.uniq[k];
+    e.l = L.u_link[L.u_ptr[k]];
+    e.c1 = L.cell1d[e.l];
+    const int l = e.l, c1 = e.c1;
+    e.h2 = (double)reinterpret_cast<const R2*>(a.h)[e.q];
+    e.hu = reinterpret_cast<const R2*>(a.hu)[e.q];
+    e.hv = reinterpret_cast<const R2*>(a.hv)[e.q];
+    const Sec s = cell_sec(N, b, c1);
+    e.A1 = a.A[(int64_t)b * N.nc + c1];
+    e.dx1 = N.dx[c1];
+    const double eta2 = L.z2[l] + e.h2;
+    const double y1 = sec_depth(s, e.A1);
+    const double eta1 = N.z[(int64_t)b * N.bs + c1] + y1;
+    const double up = tmax(eta1, eta2), dn = tmin(eta1, eta2);
+    const double d_up = clamp0(up - L.zbank[l]), d_dn = clamp0(dn - L.zbank[l]);
+    const double ratio = d_up > 0.0 ? d_dn / (d_up > 0.0 ? d_up : 1.0) : 0.0;
+    e.qw = L.weir_cw * L.length[l] * (d_up * sqrt(d_up)) * pow(clamp0(1.0 - ratio * sqrt(ratio)), 0.385);
+    e.sg = eta2 > eta1 ? 1.0 : -1.0;
+    const double plan1 = (y1 <= s.yb ? s.b + 2 * s.m * y1 : s.ts) * e.dx1;
+    e.v_eq = (up - dn) / (1.0 / L.area2 + 1.0 / plan1);
+    e.av1 = L.limit * clamp0(e.A1 - bank_area(s)) * e.dx1;
+}
+
+template <typename R2>
+__device__ __forceinline__ void ex_one_post(const ExchangeArgs& a, const int b, const int k, const ExPre<R2>& e,
+                                            const double dt) {
+    const his_links& L = a.L;
+    ExWs w = ex_ws(a, b);
+    const double V = tmin(e.qw * dt, L.relax * e.v_eq);
+    w.sg[e.l] = e.sg;
+    w.V[e.l] = V;
+    w.av1[e.c1] = e.av1;
+    w.hu_[k] = e.h2;
+    // ex_f2 and ex_f1, each over its one link
+    const double fk = ex_factor(0.0 + (e.sg > 0.0 ? V : 0.0), L.limit * e.h2 * L.area2);
+    const double f1 = ex_factor(0.0 + (e.sg < 0.0 ? V : 0.0), e.av1);
+    const double v = (e.sg > 0.0 ? V * fk : V * f1) * e.sg;
+    w.Vf[e.l] = v;
+    if (a.V) a.V[(int64_t)b * L.nl + e.l] = v;
+    const double dV = 0.0 + v;
+    a.A[(int64_t)b * a.N.nc + e.c1] = e.A1 + (0.0 + v) / e.dx1;
+    const double hn = e.h2 - dV / L.area2;
+    const double keep = e.h2 > 0.0 ? tmin(hn / (e.h2 > 0.0 ? e.h2 : 1.0), 1.0) : 0.0;
+    const R2 kp = (R2)keep;
+    reinterpret_cast<R2*>(a.h)[e.q] = (R2)hn;
+    reinterpret_cast<R2*>(a.hu)[e.q] = e.hu * kp;
+    reinterpret_cast<R2*>(a.hv)[e.q] = e.hv * kp;
+}
+
 // Exchange phase 2: net volume of unique 2-D cell r (r < nu) -> depth, momentum kept by
 // min(h_new / h, 1); of 1-D cell r - nu -> area.  Each link's volume is scaled by its donor's limit
 // factor, formed here from phase 1's volumes; sums run over the target's links in link order.

@@ -34,21 +34,40 @@ struct HeadState {
     int go, nsub, clamp, err;
 };
 
-// dt of one shared-dt group from its members' CFL rates, for every thread of the block (up to 256
-// threads; two workgroup barriers).  Every block that evaluates a group gets the same values.
-__device__ inline void head_group_dt(const DtArgs& a, const DtGroup& G, double (*red)[3], HeadState& hs) {
+// What head_group_dt reads: this thread's share of the group's CFL rates, and the clock of the
+// group's first member.  Issued apart from the reduction, so that a caller can keep other loads in
+// flight with them.
+struct HeadLoads {
+    double r2, r1, w1;
+    double t, t_limit, t_end;
+    int err0;
+};
+
+__device__ inline void head_loads(const DtArgs& a, const DtGroup& G, HeadLoads& v) {
     const int gi = G.gi;
-    double r2 = 0.0, r1 = 0.0, w1 = 0.0;
+    v.r2 = 0.0; v.r1 = 0.0; v.w1 = 0.0;
     for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) {
-        r2 = a.rate2d[m] > r2 ? a.rate2d[m] : r2;
+        v.r2 = a.rate2d[m] > v.r2 ? a.rate2d[m] : v.r2;
         if (a.has_net[gi]) {
-            r1 = a.rate1d[m] > r1 ? a.rate1d[m] : r1;
-            w1 = a.wet1d[m] ? 1.0 : w1;
+            v.r1 = a.rate1d[m] > v.r1 ? a.rate1d[m] : v.r1;
+            v.w1 = a.wet1d[m] ? 1.0 : v.w1;
         }
     }
-    r2 = wave_max(r2);
-    r1 = wave_max(r1);
-    w1 = wave_max(w1);
+    const int m0 = G.m0;
+    v.t = a.ctl.t[m0];
+    v.t_limit = a.ctl.t_limit[m0];
+    v.t_end = a.ctl.t_end[m0];
+    v.err0 = a.ctl.error[m0];
+}
+
+// dt of one shared-dt group from its members' CFL rates, for every thread of the block (up to 256
+// threads; two workgroup barriers).  Every block that evaluates a group gets the same values.
+__device__ inline void head_reduce(const DtArgs& a, const DtGroup& G, const HeadLoads& v, double (*red)[3],
+                                   HeadState& hs) {
+    const int gi = G.gi;
+    double r2 = wave_max(v.r2);
+    double r1 = wave_max(v.r1);
+    double w1 = wave_max(v.w1);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) { red[w][0] = r2; red[w][1] = r1; red[w][2] = w1; }
     __syncthreads();
@@ -58,9 +77,8 @@ __device__ inline void head_group_dt(const DtArgs& a, const DtGroup& G, double (
             r1 = red[k][1] > r1 ? red[k][1] : r1;
             w1 = red[k][2] > w1 ? red[k][2] : w1;
         }
-        const int m0 = G.m0;
-        const double t = a.ctl.t[m0];
-        const double lim = a.ctl.t_limit[m0] < a.ctl.t_end[m0] ? a.ctl.t_limit[m0] : a.ctl.t_end[m0];
+        const double t = v.t;
+        const double lim = v.t_limit < v.t_end ? v.t_limit : v.t_end;
         const int act = t < lim - 1e-9;
         double dt = 0.0;
         int nsub = 0, clamp = 0, err = 0;
@@ -88,19 +106,28 @@ __device__ inline void head_group_dt(const DtArgs& a, const DtGroup& G, double (
         hs.nsub = nsub;
         hs.clamp = clamp;
         hs.err = err;
-        // a member flagged before stays stopped; a flag raised now stops the whole group
-        hs.go = act && !err && a.ctl.error[m0] == 0;
+        // a member flagged before stays stopped; a flag raised now stops the whole group.  Blocks of
+        // one launch that evaluate the same group compute the same err, and a block writes a flag
+        // only when err != 0, when go is false whatever err0 held.
+        hs.go = act && !err && v.err0 == 0;
     }
     __syncthreads();
 }
 
+__device__ inline void head_group_dt(const DtArgs& a, const DtGroup& G, double (*red)[3], HeadState& hs) {
+    HeadLoads v;
+    head_loads(a, G, v);
+    head_reduce(a, G, v, red, hs);
+}
+
 // member m's control entries for the step (one thread per member)
-__device__ inline void head_member_ctl(const DtArgs& a, const int m, const HeadState& hs, double* rate_reset) {
+__device__ inline void head_member_ctl(const DtArgs& a, const int m, const HeadState& hs, double* rate_reset,
+                                       const double t) {
     if (hs.err) a.ctl.error[m] = hs.err;
     a.active[m] = hs.go;
     if (!hs.go) return;
     const double dt = hs.dt;
-    a.t_step[m] = a.ctl.t[m];
+    a.t_step[m] = t;
     a.ctl.dt[m] = dt;
     a.ctl.n_sub[m] = hs.nsub;
     if (hs.clamp) a.ctl.clamped[m] += 1;
@@ -111,9 +138,10 @@ __device__ inline void head_member_ctl(const DtArgs& a, const int m, const HeadS
 }
 
 // forcing slot of member m for the step: 0 rain at t, 1-4 W, E, S, N at t, 5-8 the same at t + dt
-__device__ inline void head_member_pm(const DtArgs& a, const int gi, const int m, const int slot, const double dt) {
+// (t: the member's clock, ctl.t[m])
+__device__ inline void head_member_pm(const DtArgs& a, const int gi, const int m, const int slot, const double dt,
+                                      const double t) {
     const int b = m - a.member0[gi];
-    const double t = a.ctl.t[m];
     double v = 0.0;
     if (slot == 0) {
         v = series_eval(a.rain[gi], b, t);
