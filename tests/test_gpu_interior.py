@@ -4,8 +4,9 @@ no grid edge, against the float64 oracle.
 The register-marching kernels take a leaner path for waves whose strip and rows lie wholly inside
 the grid; the reference fixtures (48 x 40) are too small to contain one, so this case supplies it:
 190 x 130, two members, wet and dry cells with fronts, rain, Horton infiltration, a retention
-store and all four edge kinds, two steps.  float64 is held to 1e-9 relative of the oracle and
-float32 to float32 round-off.
+store and all four edge kinds, two steps; a second set of edges swaps the kinds between the x and
+y sides, so that each kind is met both by strips at a west or east edge and by row chunks at a south
+or north edge.  float64 is held to 1e-9 relative of the oracle and float32 to float32 round-off.
 """
 import numpy as np
 import pytest
@@ -16,10 +17,15 @@ from _helpers import assert_rel
 pytestmark = pytest.mark.gpu
 RTOL, ATOL = 1e-9, 1e-13
 NY, NX, B = 130, 190, 2
-BCS = dict(west="transmissive", east="stage", south="discharge", north="closed")
+# edge kinds, and the forcing rows (W, E, S, N at t and at t + dt) their valued edges take
+BCSETS = {
+    "we_transmissive_stage": dict(west="transmissive", east="stage", south="discharge", north="closed"),
+    "we_discharge_closed": dict(west="discharge", east="closed", south="transmissive", north="stage"),
+}
+EDGE_VALUES = {"stage": (2.1, 1.8), "discharge": (0.4, 0.7)}
 
 
-def _case():
+def _case(bcset):
     rng = np.random.default_rng(20261203)
     yy, xx = np.mgrid[0:NY, 0:NX].astype(np.float64)
     z = 1.0 + 0.012 * xx - 0.006 * yy + 0.6 * np.sin(xx / 9.0) * np.cos(yy / 7.0)
@@ -36,10 +42,16 @@ def _case():
     fc = np.full((B, NY, NX), 10.0 / 3.6e6)
     k = np.full((B, NY, NX), 2.0 / 3600.0)
     ic = np.zeros((B, NY, NX))
-    # rain, W, E, S, N at t and at t + dt (stage on the east edge, inflow per metre on the south edge)
-    pm = np.array([[40.0 / 3.6e6, 60.0 / 3.6e6], [0.0, 0.0], [2.1, 1.8], [0.4, 0.7], [0.0, 0.0],
-                   [0.0, 0.0], [2.15, 1.85], [0.45, 0.75], [0.0, 0.0]])
-    return dict(z=z, n=n, S=S, f0=f0, fc=fc, k=k, state=[h, hu, hv, s, ic], pm=pm, dx=20.0)
+    # rain, W, E, S, N at t and at t + dt (stage levels in m, inflow per metre in m^2/s)
+    bcs = BCSETS[bcset]
+    pm = np.zeros((9, 2))
+    pm[0] = [40.0 / 3.6e6, 60.0 / 3.6e6]
+    for e, name in enumerate(("west", "east", "south", "north")):
+        v = EDGE_VALUES.get(bcs[name])
+        if v:
+            pm[1 + e] = v
+            pm[5 + e] = (v[0] + 0.05, v[1] + 0.05)
+    return dict(z=z, n=n, S=S, f0=f0, fc=fc, k=k, state=[h, hu, hv, s, ic], pm=pm, dx=20.0, bcs=bcs)
 
 
 def _sim(c, gpu, dtype):
@@ -47,20 +59,20 @@ def _sim(c, gpu, dtype):
     from his_amd.solver.boundaries import EdgeBC
     from his_amd.solver.infiltration import Horton
     from his_amd.solver.swe2d import SWE2D
-    bcs = {"west": EdgeBC("transmissive"), "east": EdgeBC("stage", 2.0), "south": EdgeBC("discharge", 0.5),
-           "north": EdgeBC("closed")}
+    value = {"stage": 2.0, "discharge": 0.5}
+    bcs = {e: EdgeBC(kind, value[kind]) if kind in value else EdgeBC(kind) for e, kind in c["bcs"].items()}
     T = lambda a: torch.as_tensor(a, dtype=dtype, device=gpu)
     hz = Horton(T(c["f0"]), T(c["fc"]), T(c["k"]), 0.0)
     return SWE2D(c["z"], c["n"], c["dx"], SolverConfig(), bcs, gpu, dtype, storage=c["S"], horton=hz, batch=B)
 
 
-@pytest.fixture(scope="module")
-def reference():
+@pytest.fixture(scope="module", params=sorted(BCSETS))
+def reference(request):
     """Two oracle steps from the case's state: (states[0..2], dt[0..1], vols[0..1], member CFL rates of
     the first state)."""
     from oracle.oracle import Oracle2D
-    c = _case()
-    orc = Oracle2D(c["z"], c["n"], c["dx"], BCS, B=B, storage=c["S"], horton=(c["f0"], c["fc"], c["k"], 0.0))
+    c = _case(request.param)
+    orc = Oracle2D(c["z"], c["n"], c["dx"], c["bcs"], B=B, storage=c["S"], horton=(c["f0"], c["fc"], c["k"], 0.0))
     states, dts, vols = [c["state"]], [], []
     rate0 = orc.max_rate(*c["state"][:3])
     t = 600.0
