@@ -11,7 +11,6 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <algorithm>
 #include <vector>
 
 #include "finalize_dev.hpp"
@@ -321,7 +320,7 @@ __global__ void __launch_bounds__(256) k_dt(const DtArgs a) {
     __shared__ double red[4][3];
     __shared__ HeadState hs;
     head_group_dt(a, G, red, hs);
-    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) head_member_ctl(a, m, hs, a.rate2d_reset, a.ctl.t[m], member_seen(a, m));
+    for (int m = G.m0 + threadIdx.x; m < G.m1; m += blockDim.x) head_member_ctl(a, m, hs, a.rate2d_reset, a.ctl.t[m]);
     if (!hs.go) return;
     const int nm = G.m1 - G.m0;
     for (int k = threadIdx.x; k < nm * 9; k += blockDim.x)
@@ -355,9 +354,6 @@ struct his_engine {
     int ex_one;               // every 1-D cell has at most one link: the exchange is one launch
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
-    ExHeadBlock* hblk;        // fused head with one phase: per item block (device)
-    DtGroup dtg0;             // the first dt group
-    ExOneCell* cells[HIS_MAX_GROUPS];   // and per unique linked cell of each group (device)
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
     int kmulti;
     int graphs_ready;
@@ -416,13 +412,6 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
     if (e->fuse_head) {
         X.head = e->head_dev;
         X.dtg_of = e->dtg_of;
-        if (e->ex_one) {
-            X.hin = head_in_of(e->D);
-            X.hblk = e->n_dtg == 1 ? nullptr : e->hblk;
-            X.dtg0 = e->dtg0;
-            for (int gi = 0, k = 0; gi < e->d.ngroups; ++gi)
-                if (e->has_net[gi]) X.cells[k++] = e->cells[gi];
-        }
         // the second phase clears the rates once every block has read them; with a single phase
         // stage 1 of the 2-D step does it (Launch2D::rate_reset)
         X.rate_reset = e->ex_one ? nullptr : e->rate2d;
@@ -459,22 +448,6 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
     return 0;
 }
 
-// Tables of the fused step head with the single-phase exchange: one ExOneCell per unique linked
-// 2-D cell of each group, one ExHeadBlock per 256 items (B x nu over the groups).  Reserved whether
-// or not the engine ends up using them (that depends on the link tables' contents).
-static int64_t ex_one_items(const his_engine_desc* d) {
-    int64_t n = 0;
-    for (int gi = 0; gi < d->ngroups; ++gi)
-        if (d->group[gi].net && d->group[gi].links) n += (int64_t)d->group[gi].grid.B * d->group[gi].links->nu;
-    return n;
-}
-static size_t ex_tables_bytes(const his_engine_desc* d) {
-    size_t tot = align_up((size_t)((ex_one_items(d) + 255) / 256) * sizeof(ExHeadBlock));
-    for (int gi = 0; gi < d->ngroups; ++gi)
-        if (d->group[gi].net && d->group[gi].links) tot += align_up((size_t)d->group[gi].links->nu * sizeof(ExOneCell));
-    return tot;
-}
-
 extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
     if (!d || d->ngroups < 1 || d->ngroups > HIS_MAX_GROUPS) return 0;
     size_t tot = 0;
@@ -496,7 +469,6 @@ extern "C" size_t his_engine_workspace_bytes(const his_engine_desc* d) {
            align_up((size_t)Btot * 2 * 8) + align_up((size_t)Btot * 4) + align_up((size_t)Btot * 3 * 8) +
            align_up((size_t)Btot * sizeof(DtGroup)) + align_up(sizeof(DtArgs)) + align_up(sizeof(FinalizeArgs)) +
            align_up((size_t)Btot * 4);
-    tot += ex_tables_bytes(d);
     return tot;
 }
 
@@ -515,35 +487,6 @@ static void engine_fill_launch(his_engine* e, Launch2D<R>& L, char*& p) {
     int tx, ty;
     e->shape = shape_of(e->dtype, &tx, &ty);
     tiles_for(L, tx, ty);
-}
-
-// ExOneCell of every unique linked cell of a link table, ordered by the 1-D cell of its first link
-// (read back from the device; runs once per engine)
-static int ex_one_cells(const his_links& L, ExOneCell* dev, hipStream_t s) {
-    if (L.nu < 1) return 0;
-    std::vector<int64_t> uniq(L.nu);
-    std::vector<int32_t> u_ptr((size_t)L.nu + 1), u_link(L.nl), cell1d(L.nl);
-    hipError_t he = hipMemcpyAsync(uniq.data(), L.uniq, uniq.size() * 8, hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess) he = hipMemcpyAsync(u_ptr.data(), L.u_ptr, u_ptr.size() * 4, hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess) he = hipMemcpyAsync(u_link.data(), L.u_link, u_link.size() * 4, hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess) he = hipMemcpyAsync(cell1d.data(), L.cell1d, cell1d.size() * 4, hipMemcpyDeviceToHost, s);
-    if (he == hipSuccess) he = hipStreamSynchronize(s);
-    if (he != hipSuccess) return hip_fail(he, "reading the link table");
-    std::vector<ExOneCell> cells(L.nu);
-    for (int k = 0; k < L.nu; ++k) {
-        if (u_ptr[k + 1] <= u_ptr[k] || u_ptr[k] < 0 || u_ptr[k + 1] > L.nl) {
-            set_error("link table: unique cell %d has no link", k);
-            return HIS_E_ARG;
-        }
-        const int l = u_link[u_ptr[k]];
-        if (l < 0 || l >= L.nl) { set_error("link table: link index %d out of range", l); return HIS_E_ARG; }
-        cells[k] = ExOneCell{uniq[k], l, cell1d[l], k, u_ptr[k + 1] - u_ptr[k] == 1};
-    }
-    std::stable_sort(cells.begin(), cells.end(), [](const ExOneCell& x, const ExOneCell& y) { return x.c1 < y.c1; });
-    he = hipMemcpyAsync(dev, cells.data(), cells.size() * sizeof(ExOneCell), hipMemcpyHostToDevice, s);
-    if (he == hipSuccess) he = hipStreamSynchronize(s);
-    if (he != hipSuccess) return hip_fail(he, "writing the exchange item table");
-    return 0;
 }
 
 extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, void* stream) {
@@ -619,13 +562,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     DtGroup* dtg_dev = (DtGroup*)p; p += align_up((size_t)Btot * sizeof(DtGroup));
     e->head_dev = (DtArgs*)p; p += align_up(sizeof(DtArgs));
     e->fin_dev = (FinalizeArgs*)p; p += align_up(sizeof(FinalizeArgs));
-    e->dtg_of = (int32_t*)p; p += align_up((size_t)Btot * 4);
-    e->hblk = (ExHeadBlock*)p; p += align_up((size_t)((ex_one_items(d) + 255) / 256) * sizeof(ExHeadBlock));
-    for (int gi = 0; gi < d->ngroups; ++gi) {
-        if (!d->group[gi].net) continue;
-        e->cells[gi] = (ExOneCell*)p;
-        p += align_up((size_t)d->group[gi].links->nu * sizeof(ExOneCell));
-    }
+    e->dtg_of = (int32_t*)p;
     // launches; with a network in every group the 1-D kernel advances t, so the ledger reduction is
     // off the step's critical path and the last stage publishes the CFL rates itself
     e->async_fin = 1;
@@ -721,41 +658,6 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(step head)"); }
     he = hipMemsetAsync(e->snap, 0, (size_t)Btot * 3 * 8, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemsetAsync(ledger snapshot)"); }
-    std::vector<ExHeadBlock> hb;
-    e->dtg0 = dtg[0];
-    if (e->fuse_head && e->ex_one) {
-        // the fused launch's items, group after group (groups with a network, in index order), B x nu
-        // each; block k holds items 256 k .. 256 k + 255
-        std::vector<int64_t> it0{0};
-        std::vector<int> nus, m0s;
-        for (int gi = 0; gi < d->ngroups; ++gi) {
-            if (!e->has_net[gi]) continue;
-            it0.push_back(it0.back() + (int64_t)d->group[gi].grid.B * e->links[gi].nu);
-            nus.push_back(e->links[gi].nu);
-            m0s.push_back(dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0);
-            if (const int rc = ex_one_cells(e->links[gi], e->cells[gi], s)) { delete e; return rc; }
-        }
-        const int64_t total = it0.back();
-        auto member_of = [&](int64_t it, int& first) {
-            size_t g = 0;
-            while (g + 1 < nus.size() && it >= it0[g + 1]) ++g;
-            const int64_t loc = it - it0[g];
-            first = loc % nus[g] == 0;
-            return m0s[g] + (int)(loc / nus[g]);
-        };
-        for (int64_t i0 = 0; i0 < total; i0 += 256) {
-            ExHeadBlock x{};
-            int fB;
-            x.mA = member_of(i0, x.firstA);
-            x.mB = member_of(i0 + 255 < total - 1 ? i0 + 255 : total - 1, fB);
-            x.two = x.mB != x.mA;
-            x.gA = dtg[dtg_of[x.mA]];
-            x.gB = dtg[dtg_of[x.mB]];
-            hb.push_back(x);
-        }
-        he = hipMemcpyAsync(e->hblk, hb.data(), hb.size() * sizeof(ExHeadBlock), hipMemcpyHostToDevice, s);
-        if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(exchange head blocks)"); }
-    }
     he = hipStreamSynchronize(s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipStreamSynchronize"); }
     if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess ||

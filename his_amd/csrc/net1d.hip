@@ -384,70 +384,55 @@ __device__ __forceinline__ int ex_link_member(const ExGroups& G, int64_t it, boo
     return a.member0 + b;
 }
 
-// The step head fused with the single-phase exchange (one item per unique 2-D cell).  Both read
-// through chains of dependent loads: the head from its members' dt groups to their rates and clocks,
-// the item from its cell to the link, the link's 1-D cell and that cell's section and area.  The
-// head's inputs sit in the kernel arguments and the engine's tables give each block its members
-// and dt groups (ExHeadBlock) and each item its cell, link and 1-D cell (ExOneCell), so each chain
-// is two loads deep; the item's dt-free part (ex_one_pre) and the members' old control entries are
-// read between the head's loads and its reduction, so the chains are in flight together.
+// The step head fused with the single-phase exchange (one item per unique 2-D cell).  Both have a
+// chain of dependent loads: the head's from the block's members to their dt group and its rates and
+// clock, the item's from the cell to its link, its 1-D cell and that cell's section and area.  The
+// item's dt-free part (ex_one_pre) is written between the head's loads and its reduction, so the two
+// chains are in flight together; only ex_one_post waits for dt.
 template <typename R2>
 __device__ __forceinline__ void ex_head_one(const ExGroups& G, const int64_t blk, double (*red)[3], HeadState* hs) {
     const DtArgs& D = *G.head;
     const int64_t total = G.it_links[G.ngroups];
     const int64_t i0 = blk * blockDim.x;
-    ExHeadBlock hb;
-    if (G.hblk) {
-        hb = G.hblk[blk];
-    } else {
-        // one dt group: the block's members from the item numbering alone
-        const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
-        bool fA, fB;
-        hb.mA = ex_link_member(G, i0, fA);
-        hb.mB = ex_link_member(G, i1, fB);
-        hb.firstA = fA;
-        hb.two = hb.mB != hb.mA;
-        hb.gA = G.dtg0;
-        hb.gB = G.dtg0;
-    }
-    // this thread's item; threads past the last item read the last item's data and write nothing
+    const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
+    bool firstA, firstB;
+    const int mA = ex_link_member(G, i0, firstA);
+    const int mB = ex_link_member(G, i1, firstB);
+    const bool two = mB != mA;
+    // this thread's item; threads past the last item load the last item's data and write nothing
     const bool mine = i0 + (int64_t)threadIdx.x < total;
     const int64_t it = mine ? i0 + (int64_t)threadIdx.x : total - 1;
     const int gi = ex_group(G, G.it_links, it);
     const ExchangeArgs& a = G.a[gi];
     const int64_t loc = it - G.it_links[gi];
-    const int b = (int)(loc / a.L.nu), r = (int)(loc - (int64_t)b * a.L.nu);
-    const ExOneCell cell = G.cells[gi][r];
+    const int b = (int)(loc / a.L.nu), k = (int)(loc - (int64_t)b * a.L.nu);
+    const bool single = a.L.u_ptr[k + 1] - a.L.u_ptr[k] == 1;
+    const DtGroup GA = D.groups[G.dtg_of[mA]];
+    const DtGroup GB = D.groups[G.dtg_of[mB]];
     HeadLoads vA, vB;
-    head_loads(G.hin, hb.gA, vA);
-    if (hb.two) head_loads(G.hin, hb.gB, vB);
-    // the members whose control entries and forcing this block writes: their clocks and (thread 0)
-    // the entries updated from their old values
-    double tm[2] = {0.0, 0.0};
-    MemberSeen seen[2] = {};
-    for (int j = 0; j < 2; ++j) {
-        if (!(j == 0 ? hb.firstA : hb.two)) continue;
-        const int mm = j == 0 ? hb.mA : hb.mB;
-        tm[j] = G.hin.t[mm];
-        if (threadIdx.x == 0) seen[j] = member_seen(D, mm);
-    }
+    head_loads(D, GA, vA);
+    if (two) head_loads(D, GB, vB);
+    // the clocks of the members whose control entries and forcing this block writes
+    const double tA = D.ctl.t[mA], tB = D.ctl.t[mB];
     ExPre<R2> e;
-    ex_one_pre<R2>(a, b, cell, e);     // a cell with several links redoes its item after the head
-    head_reduce(vA, red, hs[0]);
-    if (hb.two) head_reduce(vB, red, hs[1]);
+    ex_one_pre<R2>(a, b, k, e);        // a cell with several links redoes its item after the head
+    head_reduce(D, GA, vA, red, hs[0]);
+    if (two) head_reduce(D, GB, vB, red, hs[1]);
     for (int j = 0; j < 2; ++j) {
-        if (!(j == 0 ? hb.firstA : hb.two)) continue;
-        const int mm = j == 0 ? hb.mA : hb.mB;
+        const int mm = j == 0 ? mA : mB;
+        const bool writes = j == 0 ? firstA : two;
+        if (!writes) continue;
         const HeadState& h = hs[j];
-        if (threadIdx.x == 0) head_member_ctl(D, mm, h, nullptr, tm[j], seen[j]);
-        if (h.go && threadIdx.x >= 64 && threadIdx.x < 73)
-            head_member_pm(D, (j == 0 ? hb.gA : hb.gB).gi, mm, threadIdx.x - 64, h.dt, tm[j]);
+        const int gim = (j == 0 ? GA : GB).gi;
+        const double tm = j == 0 ? tA : tB;
+        if (threadIdx.x == 0) head_member_ctl(D, mm, h, nullptr, tm);
+        if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(D, gim, mm, threadIdx.x - 64, h.dt, tm);
     }
     if (!mine) return;
-    const HeadState& h = a.member0 + b == hb.mA ? hs[0] : hs[1];
+    const HeadState& h = a.member0 + b == mA ? hs[0] : hs[1];
     if (!h.go) return;
-    if (cell.single) ex_one_post<R2>(a, b, cell.k, e, h.dt);
-    else ex_one_item<R2>(a, b, cell.k, h.dt);
+    if (single) ex_one_post<R2>(a, b, k, e, h.dt);
+    else ex_one_item<R2>(a, b, k, h.dt);
 }
 
 template <typename R2>
@@ -487,7 +472,7 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
             const HeadState& h = hs[j];
             const int gim = G.head->groups[G.dtg_of[mm]].gi;
             const double tm = G.head->ctl.t[mm];
-            if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr, tm, member_seen(*G.head, mm));
+            if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr, tm);
             if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(*G.head, gim, mm, threadIdx.x - 64, h.dt, tm);
         }
     }

@@ -55,35 +55,6 @@ int launch_exchange(const ExchangeArgs& a, hipStream_t s);
 struct DtArgs;
 struct FinalizeArgs;
 
-// A shared-dt group of members m0 .. m1-1 of grid group gi.
-struct DtGroup { int m0, m1, gi, pad; };
-
-// Tables of the fused step head with the single-phase exchange, fixed for an engine's life.
-// Per item block: the members of its first and last items and their dt groups.
-struct ExHeadBlock {
-    int mA, mB;
-    int firstA;              // the block holds mA's first item
-    int two;                 // mB != mA (the block then holds mB's first item too)
-    DtGroup gA, gB;
-};
-// Per item of a group: unique 2-D cell k with its flat index q, its first link l and that link's
-// 1-D cell c1, and whether l is its only link.  Items run in the order of c1, so that neighbouring
-// threads read neighbouring entries of the 1-D arrays.
-struct ExOneCell {
-    int64_t q;
-    int l, c1, k, single;
-};
-// The step head's inputs, held in the kernel arguments of the fused launch (no pointer to follow
-// before the member arrays can be read).
-struct HeadIn {
-    const double *rate2d, *rate1d, *t, *t_limit, *t_end;
-    const int32_t *wet1d, *error;
-    double cfl, dt_min, dt_max;
-    double cfl_1d[HIS_MAX_GROUPS], dt_max_1d[HIS_MAX_GROUPS];
-    int has_net[HIS_MAX_GROUPS];
-    int n_sub_limit;
-};
-
 // Several groups' exchanges in one launch per phase (every group shares the 2-D dtype).
 struct ExGroups {
     int ngroups;
@@ -102,12 +73,6 @@ struct ExGroups {
     // every group's 1-D cells have at most one link each: the first launch does the whole exchange
     // (one item per unique 2-D cell) and no second phase runs
     int one_phase;
-    // with head and one_phase: the head's inputs; ExHeadBlock per item block (NULL when one dt group
-    // holds every member: dtg0); ExOneCell per item of each group
-    HeadIn hin;
-    const ExHeadBlock* hblk;
-    DtGroup dtg0;
-    const ExOneCell* cells[HIS_MAX_GROUPS];
 };
 int launch_exchange_groups(ExGroups& g, hipStream_t s);
 

@@ -189,12 +189,12 @@ struct ExPre {
 };
 
 template <typename R2>
-__device__ __forceinline__ void ex_one_pre(const ExchangeArgs& a, const int b, const ExOneCell& cell, ExPre<R2>& e) {
+__device__ __forceinline__ void ex_one_pre(const ExchangeArgs& a, const int b, const int k, ExPre<R2>& e) {
     const his_links& L = a.L;
     const his_net1d& N = a.N;
-    e.q = (int64_t)b * a.plane + cell.q;
-    e.l = cell.l;
-    e.c1 = cell.c1;
+    e.q = (int64_t)b * a.plane + L.uniq[k];
+    e.l = L.u_link[L.u_ptr[k]];
+    e.c1 = L.cell1d[e.l];
     const int l = e.l, c1 = e.c1;
     e.h2 = (double)reinterpret_cast<const R2*>(a.h)[e.q];
     e.hu = reinterpret_cast<const R2*>(a.hu)[e.q];
