@@ -361,6 +361,9 @@ int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, doub
 //      kept by min(h_new/h, 1); per 1-D cell: the same -> new area (sums run over each target's
 //      links in link order; limits read only phase 1's outputs, so in-place updates do not feed them)
 //   3. per member: gross volumes each way into the ledger (fixed-order block reduction)
+// When every 1-D cell has at most one link, a linked 2-D cell's links and the cells they change
+// belong to it alone: one item per unique 2-D cell then does phases 1 and 2 in the first launch
+// (ex_one_item, or ex_one_pre / ex_one_post around the fused step head), with the same operations.
 
 size_t exchange_ws_doubles(const his_links& l, const his_net1d& n) {
     return (size_t)n.B * (size_t)exchange_ws_stride(l, n);
