@@ -357,6 +357,7 @@ struct his_engine {
     hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
     int kmulti;
     int graphs_ready;
+    void* step_max[HIS_MAX_GROUPS];   // per grid group: running maximum of h folded every step, or NULL
 };
 
 // Whether a step's ledger reduction is still owed.  With async_fin it is not launched after the
@@ -433,6 +434,12 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
     if (e->one_first && (rc = one_d())) return rc;
     rc = e->dtype == HIS_F64 ? launch_step(e->L64, e->shape, s) : launch_step(e->L32, e->shape, s);
     if (rc) return rc;
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        if (!e->step_max[gi]) continue;
+        rc = e->dtype == HIS_F64 ? launch_fold_max(e->L64, gi, (double*)e->step_max[gi], s)
+                                 : launch_fold_max(e->L32, gi, (float*)e->step_max[gi], s);
+        if (rc) return rc;
+    }
     if (!e->one_first && (rc = one_d())) return rc;
     if (e->async_fin) {
         // the next step's dt reads the rates the last stage folded in; the ledger reduction rides
@@ -758,12 +765,26 @@ extern "C" int his_engine_fold_max(his_engine* e, int32_t group, void* hmax, voi
     return launch_fold_max(e->L32, group, (float*)hmax, s);
 }
 
+// the captured steps no longer match the engine's settings: capture again at the next step
+static void engine_drop_graphs(his_engine* e) {
+    if (!e->graphs_ready) return;
+    (void)hipGraphExecDestroy(e->exec[0]);
+    (void)hipGraphExecDestroy(e->exec[1]);
+    e->graphs_ready = 0;
+}
+
+extern "C" int his_engine_track_step_max(his_engine* e, int32_t group, void* hmax) {
+    if (!e || group < 0 || group >= e->d.ngroups) { set_error("his_engine_track_step_max: bad argument"); return HIS_E_ARG; }
+    if (e->step_max[group] != hmax) {
+        e->step_max[group] = hmax;
+        engine_drop_graphs(e);
+    }
+    return 0;
+}
+
 extern "C" int his_engine_destroy(his_engine* e) {
     if (!e) return 0;
-    if (e->graphs_ready) {
-        (void)hipGraphExecDestroy(e->exec[0]);
-        (void)hipGraphExecDestroy(e->exec[1]);
-    }
+    engine_drop_graphs(e);
     (void)hipStreamDestroy(e->cap);
     (void)hipStreamDestroy(e->side);
     (void)hipEventDestroy(e->ev_fork);

@@ -233,6 +233,13 @@ int his_engine_refresh_members(his_engine *e, const int32_t *mask, void *stream)
  * depth_max of FloodResult taken over output times (hydrointel/api.py:239). */
 int his_engine_fold_max(his_engine *e, int32_t group, void *hmax, void *stream);
 
+/* From the next step on, fold group `group`'s depth into hmax (device, (B, ny, nx), grid dtype)
+ * after the 2-D stages of every coupled step; hmax = NULL stops it.  The captured step graphs
+ * are captured again at the next step.  No counterpart in hydrointel, whose depth_max is taken
+ * over output times only (hydrointel/api.py:239): for callers that need the peak between outputs.
+ * Cost: one pass over the group's depth per step. */
+int his_engine_track_step_max(his_engine *e, int32_t group, void *hmax);
+
 int his_engine_destroy(his_engine *e);
 
 /* ------------------------------------------------------------------ misc */
