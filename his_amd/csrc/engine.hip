@@ -717,18 +717,23 @@ static int engine_build_graphs(his_engine* e) {
     e->kmulti = 8;
     for (int k = 0; k < 2; ++k) {
         const int n = k == 0 ? 1 : e->kmulti;
-        hipGraph_t g;
+        hipGraph_t g = nullptr;
         HIS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));
         int rc = 0;
         FinChain chain;
         for (int i = 0; i < n && !rc; ++i) rc = engine_enqueue_step(e, e->cap, e->side, chain);
         if (!rc) rc = engine_close_chain(e, e->cap, chain);
         hipError_t he = hipStreamEndCapture(e->cap, &g);
-        if (rc) return rc;
-        if (he != hipSuccess) return hip_fail(he, "hipStreamEndCapture");
-        he = hipGraphInstantiate(&e->exec[k], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (he != hipSuccess) return hip_fail(he, "hipGraphInstantiate");
+        if (!rc && he != hipSuccess) rc = hip_fail(he, "hipStreamEndCapture");
+        if (!rc) {
+            he = hipGraphInstantiate(&e->exec[k], g, nullptr, nullptr, 0);
+            if (he != hipSuccess) rc = hip_fail(he, "hipGraphInstantiate");
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) {
+            if (k == 1) (void)hipGraphExecDestroy(e->exec[0]);   // the one-step graph made above
+            return rc;
+        }
     }
     e->graphs_ready = 1;
     return 0;
@@ -768,6 +773,7 @@ extern "C" int his_engine_fold_max(his_engine* e, int32_t group, void* hmax, voi
 // the captured steps no longer match the engine's settings: capture again at the next step
 static void engine_drop_graphs(his_engine* e) {
     if (!e->graphs_ready) return;
+    (void)hipDeviceSynchronize();   // a replay may still be running on the caller's stream
     (void)hipGraphExecDestroy(e->exec[0]);
     (void)hipGraphExecDestroy(e->exec[1]);
     e->graphs_ready = 0;
