@@ -170,8 +170,9 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             if (SL >= 0.0) { f1 = QL; f2 = FL2; }
             else if (SR <= 0.0) { f1 = QR; f2 = FR2; }
             else {
-                f1 = (SR * QL - SL * QR + SL * SR * (AR - AL)) / den;
-                f2 = (SR * FL2 - SL * FR2 + SL * SR * (QR - QL)) / den;
+                const double rden = 1.0 / den;
+                f1 = (SR * QL - SL * QR + SL * SR * (AR - AL)) * rden;
+                f2 = (SR * FL2 - SL * FR2 + SL * SR * (QR - QL)) * rden;
             }
             if (dryL && dryR) { f1 = 0.0; f2 = 0.0; }
             F1[f] = f1; F2[f] = f2; pL[f] = pl; pR[f] = pr;
@@ -213,8 +214,9 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             }
             const int kd = N.kind[c];
             if (kd >= 2) continue;   // ghosts keep their (refilled) values
-            double A1 = A[c] + dt * dA / N.dx[c];
-            double Q1 = Q[c] + dt * dQ / N.dx[c];
+            const double rdx = dt / N.dx[c];
+            double A1 = A[c] + dA * rdx;
+            double Q1 = Q[c] + dQ * rdx;
             const double neg = clamp0(-A1);
             part[0] += neg * N.dx[c];
             A1 = A1 + neg;
@@ -224,8 +226,9 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             const double Rh = Ac / sec_perim(s, y1);
             const double Rc = Rh < h_dry ? h_dry : Rh;
             const double nm = N.nman[c];
-            const double denom = 1.0 + dt * g * (nm * nm) * fabs(Q1) / (Ac * (Rc * cbrt(Rc)));
-            Q1 = Q1 / denom;
+            // Q / (1 + dt g n^2 |Q| / (A R^(4/3))) with one division: Q X / (X + dt g n^2 |Q|)
+            const double X = Ac * (Rc * cbrt(Rc));
+            Q1 = Q1 * X / (X + dt * g * (nm * nm) * fabs(Q1));
             if (!((y1 >= h_dry) && kd != 1)) Q1 = 0.0;
             A[c] = A1;
             Q[c] = Q1;

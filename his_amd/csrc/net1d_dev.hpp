@@ -26,11 +26,15 @@ __device__ __forceinline__ double sec_perim(const Sec& s, double y) {
     double yt = tmin(y, s.yb);
     return s.b + 2.0 * yt * sqrt(1.0 + s.m * s.m);
 }
+// one division for both branches: below the bank y = 0 + 2A / (b + sqrt(b^2 + 4mA)), above it
+// y = yb + (A - ab) / ts; adding 0.0 is exact, so the result is that of the two-branch form
 __device__ __forceinline__ double sec_depth(const Sec& s, double A) {
     A = clamp0(A);
-    double ab = bank_area(s);
-    double y_t = 2.0 * A / (s.b + sqrt(s.b * s.b + 4.0 * s.m * A));
-    return A <= ab ? y_t : s.yb + (A - ab) / s.ts;
+    const double ab = bank_area(s);
+    const bool below = A <= ab;
+    const double num = below ? 2.0 * A : A - ab;
+    const double den = below ? s.b + sqrt(s.b * s.b + 4.0 * s.m * A) : s.ts;
+    return (below ? 0.0 : s.yb) + num / den;
 }
 __device__ __forceinline__ Sec cell_sec(const his_net1d& N, int b, int c) {
     const int64_t o = (int64_t)b * N.bs + c;
