@@ -201,19 +201,20 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
         for (int c = threadIdx.x; c < nc; c += NT) {
+            const int kd = N.kind[c];
+            if (kd >= 2) continue;   // ghosts keep their (refilled) values
+            // dA and dQ each summed over the left faces, then the right faces, in list order
             double dA = 0.0, dQ = 0.0;
-            for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) dA += -F1[N.lf_face[p]];
-            for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) dA += F1[N.rf_face[p]];
             for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) {
                 const int f = N.lf_face[p];
+                dA += -F1[f];
                 dQ += -(F2[f] - pL[f]);
             }
             for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) {
                 const int f = N.rf_face[p];
+                dA += F1[f];
                 dQ += F2[f] - pR[f];
             }
-            const int kd = N.kind[c];
-            if (kd >= 2) continue;   // ghosts keep their (refilled) values
             const double rdx = dt / N.dx[c];
             double A1 = A[c] + dA * rdx;
             double Q1 = Q[c] + dQ * rdx;
