@@ -18,7 +18,7 @@ __device__ __forceinline__ double sec_top(const Sec& s, double y) { return y <= 
 __device__ __forceinline__ double sec_i1(const Sec& s, double y) {
     double yt = tmin(y, s.yb);
     double above = clamp0(y - s.yb);
-    double i_t = s.b * yt * yt / 2.0 + s.m * (yt * yt * yt) / 3.0;
+    double i_t = s.b * yt * yt / 2.0 + s.m * (yt * yt * yt) * (1.0 / 3.0);   // a product, not a division
     double at = yt * (s.b + s.m * yt);
     return i_t + at * above + s.ts * above * above / 2.0;
 }
