@@ -215,7 +215,8 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
                 dA += F1[f];
                 dQ += F2[f] - pR[f];
             }
-            const double rdx = dt / N.dx[c];
+            const double inv_dx = 1.0 / N.dx[c];
+            const double rdx = dt * inv_dx;
             double A1 = A[c] + dA * rdx;
             double Q1 = Q[c] + dQ * rdx;
             const double neg = clamp0(-A1);
@@ -237,7 +238,7 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
                 r_wet = 1.0;
                 const double cc = sqrt(g * A1 / sec_top(s, y1));
                 const double uu = vel1(s, A1, Q1, h_dry);
-                const double r = (fabs(uu) + cc) / N.dx[c];
+                const double r = (fabs(uu) + cc) * inv_dx;
                 r_mx = r > r_mx ? r : r_mx;
             }
         }
