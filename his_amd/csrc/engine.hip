@@ -624,11 +624,33 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
         for (int c = 0; c < nc; ++c)
             if (cp[c + 1] - cp[c] > 1) { e->ex_one = 0; break; }
     }
-    // the fused head needs every member's first-launch items to span a whole block (two members
-    // per block at most): links and unique cells, or unique cells alone with one phase
+    // the fused head: every member has first-launch items (links and unique cells, or unique cells
+    // alone with one phase), and the members of each block of 256 items lie in at most two dt groups
     e->fuse_head = e->async_fin;
-    for (int gi = 0; gi < d->ngroups; ++gi)
-        if (e->has_net[gi] && (e->ex_one ? e->links[gi].nu : e->links[gi].nl + e->links[gi].nu) < 256) e->fuse_head = 0;
+    if (e->fuse_head) {
+        std::vector<int32_t> gof(Btot, 0);
+        for (int k = 0; k < (int)dtg.size(); ++k)
+            for (int m = dtg[k].m0; m < dtg[k].m1; ++m) gof[m] = k;
+        std::vector<int64_t> it0(1, 0);
+        std::vector<int> per_g, m0_g;
+        for (int gi = 0; gi < d->ngroups; ++gi) {
+            const int per = e->ex_one ? e->links[gi].nu : e->links[gi].nl + e->links[gi].nu;
+            if (per < 1) e->fuse_head = 0;
+            per_g.push_back(per < 1 ? 1 : per);
+            m0_g.push_back(dtype == HIS_F64 ? e->L64.g[gi].member0 : e->L32.g[gi].member0);
+            it0.push_back(it0.back() + (int64_t)d->group[gi].grid.B * per);
+        }
+        auto member_of = [&](int64_t it) {
+            int gi = 0;
+            while (gi + 1 < d->ngroups && it >= it0[gi + 1]) ++gi;
+            return m0_g[gi] + (int)((it - it0[gi]) / per_g[gi]);
+        };
+        const int64_t total = it0.back();
+        for (int64_t i0 = 0; i0 < total && e->fuse_head; i0 += 256) {
+            const int64_t i1 = i0 + 255 < total - 1 ? i0 + 255 : total - 1;
+            if (gof[member_of(i1)] - gof[member_of(i0)] > 1) e->fuse_head = 0;
+        }
+    }
     D.rate2d_reset = e->async_fin && !e->fuse_head ? e->rate2d : nullptr;
     if (e->async_fin && e->ex_one) {
         if (dtype == HIS_F64) e->L64.rate_reset = e->rate2d;

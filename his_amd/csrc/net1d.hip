@@ -406,7 +406,9 @@ __device__ __forceinline__ void ex_head_one(const ExGroups& G, const int64_t blk
     bool firstA, firstB;
     const int mA = ex_link_member(G, i0, firstA);
     const int mB = ex_link_member(G, i1, firstB);
-    const bool two = mB != mA;
+    // the block's members lie in at most two dt groups (checked when the engine is created)
+    const int gA = G.dtg_of[mA], gB = G.dtg_of[mB];
+    const bool two = gB != gA;
     // this thread's item; threads past the last item load the last item's data and write nothing
     const bool mine = i0 + (int64_t)threadIdx.x < total;
     const int64_t it = mine ? i0 + (int64_t)threadIdx.x : total - 1;
@@ -415,8 +417,8 @@ __device__ __forceinline__ void ex_head_one(const ExGroups& G, const int64_t blk
     const int64_t loc = it - G.it_links[gi];
     const int b = (int)(loc / a.L.nu), k = (int)(loc - (int64_t)b * a.L.nu);
     const bool single = a.L.u_ptr[k + 1] - a.L.u_ptr[k] == 1;
-    const DtGroup GA = D.groups[G.dtg_of[mA]];
-    const DtGroup GB = D.groups[G.dtg_of[mB]];
+    const DtGroup GA = D.groups[gA];
+    const DtGroup GB = D.groups[gB];
     HeadLoads vA, vB;
     head_loads(D, GA, vA);
     if (two) head_loads(D, GB, vB);
@@ -426,18 +428,17 @@ __device__ __forceinline__ void ex_head_one(const ExGroups& G, const int64_t blk
     ex_one_pre<R2>(a, b, k, e);        // a cell with several links redoes its item after the head
     head_reduce(D, GA, vA, red, hs[0]);
     if (two) head_reduce(D, GB, vB, red, hs[1]);
-    for (int j = 0; j < 2; ++j) {
-        const int mm = j == 0 ? mA : mB;
-        const bool writes = j == 0 ? firstA : two;
-        if (!writes) continue;
+    // every member whose first item lies in this block: mA if its first item is i0, and the rest
+    for (int mm = firstA ? mA : mA + 1; mm <= mB; ++mm) {
+        const int j = mm < GA.m1 ? 0 : 1;
         const HeadState& h = hs[j];
         const int gim = (j == 0 ? GA : GB).gi;
-        const double tm = j == 0 ? tA : tB;
+        const double tm = mm == mA ? tA : (mm == mB ? tB : D.ctl.t[mm]);
         if (threadIdx.x == 0) head_member_ctl(D, mm, h, nullptr, tm);
         if (h.go && threadIdx.x >= 64 && threadIdx.x < 73) head_member_pm(D, gim, mm, threadIdx.x - 64, h.dt, tm);
     }
     if (!mine) return;
-    const HeadState& h = a.member0 + b == mA ? hs[0] : hs[1];
+    const HeadState& h = a.member0 + b < GA.m1 ? hs[0] : hs[1];
     if (!h.go) return;
     if (single) ex_one_post<R2>(a, b, k, e, h.dt);
     else ex_one_item<R2>(a, b, k, h.dt);
@@ -462,22 +463,22 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
         ex_head_one<R2>(G, blk, red, hs);
         return;
     }
-    int mA = -1, mB = -1;
+    int mA = -1, mB = -1, mA_end = 0;
     if (G.head) {
-        // a block spans at most two members (nl + nu >= 256): evaluate their groups' dt here
+        // the block's members lie in at most two dt groups (checked when the engine is created):
+        // evaluate their dt here
         const int64_t i0 = blk * blockDim.x;
         const int64_t i1 = i0 + blockDim.x - 1 < total - 1 ? i0 + blockDim.x - 1 : total - 1;
         bool firstA, firstB;
         mA = ex_link_member(G, i0, firstA);
         mB = ex_link_member(G, i1, firstB);
-        head_group_dt(*G.head, G.head->groups[G.dtg_of[mA]], red, hs[0]);
-        if (mB != mA) head_group_dt(*G.head, G.head->groups[G.dtg_of[mB]], red, hs[1]);
+        const int gA = G.dtg_of[mA], gB = G.dtg_of[mB];
+        mA_end = G.head->groups[gA].m1;
+        head_group_dt(*G.head, G.head->groups[gA], red, hs[0]);
+        if (gB != gA) head_group_dt(*G.head, G.head->groups[gB], red, hs[1]);
         // the block holding a member's first item writes its control entries and its forcing
-        for (int j = 0; j < 2; ++j) {
-            const int mm = j == 0 ? mA : mB;
-            const bool mine = j == 0 ? firstA : (mB != mA);
-            if (!mine) continue;
-            const HeadState& h = hs[j];
+        for (int mm = firstA ? mA : mA + 1; mm <= mB; ++mm) {
+            const HeadState& h = hs[mm < mA_end ? 0 : 1];
             const int gim = G.head->groups[G.dtg_of[mm]].gi;
             const double tm = G.head->ctl.t[mm];
             if (threadIdx.x == 0) head_member_ctl(*G.head, mm, h, nullptr, tm);
@@ -494,7 +495,7 @@ __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     const int m = a.member0 + b;
     double dt;
     if (G.head) {
-        const HeadState& h = m == mA ? hs[0] : hs[1];
+        const HeadState& h = m < mA_end ? hs[0] : hs[1];
         if (!h.go) return;
         dt = h.dt;
     } else {
