@@ -13,6 +13,8 @@ import torch
 LIBPATH = Path(__file__).resolve().parent / "_lib" / "libhis_amd.so"
 
 HIS_F64, HIS_F32 = 0, 1
+HIS_E_WORKSPACE = -3
+SWE2D_NOALLOC_MAX_BYTES = 64 * 1024 * 1024     # HIS_SWE2D_NOALLOC_MAX_BYTES of include/his_amd.h
 BC = {"closed": 0, "transmissive": 1, "stage": 2, "discharge": 3}
 MAX_GROUPS = 4
 VP = C.c_void_p
@@ -75,6 +77,7 @@ class EngineDesc(C.Structure):
 SYMBOLS = {
     "his_swe2d_workspace_bytes": (C.c_size_t, [C.POINTER(Grid2D)]),
     "his_swe2d_step": (C.c_int, [C.POINTER(Grid2D), State2D, VP, VP, VP, VP, VP, C.c_size_t, VP]),
+    "his_swe2d_step_noalloc": (C.c_int, [C.POINTER(Grid2D), State2D, VP, VP, VP, VP, VP]),
     "his_swe2d_max_rate": (C.c_int, [C.POINTER(Grid2D), VP, VP, VP, VP, VP, C.c_size_t, VP]),
     "his_swe2d_velocities": (C.c_int, [C.POINTER(Grid2D), VP, VP, VP, VP, VP, VP]),
     "his_swe2d_volumes": (C.c_int, [C.POINTER(Grid2D), VP, VP, VP, VP, C.c_size_t, VP]),

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "finalize_dev.hpp"
@@ -178,6 +179,65 @@ extern "C" int his_swe2d_step(const his_grid2d* g, his_state2d st, const double*
         set_error("his_swe2d_step: workspace of %zu bytes, %zu needed", ws_bytes, ws2d_bytes(*g));
         return HIS_E_WORKSPACE;
     }
+    hipStream_t s = (hipStream_t)stream;
+    return g->dtype == HIS_F64 ? op_step<double>(g, st, dt_t, pm, vols, rate, ws, s)
+                               : op_step<float>(g, st, dt_t, pm, vols, rate, ws, s);
+}
+
+// ------------------------------------------------------------------ C ABI: 2-D step without allocation
+//
+// Scratch of his_swe2d_step_noalloc: one file-scope device array, a copy of which every GPU the
+// library runs on holds from module load until the process ends.  In host code `scratch` names a
+// host-side stand-in, not a device address, so it is never passed to a kernel or a copy: the device
+// address comes from hipGetSymbolAddress, taken once per device with that device current and kept in
+// g_scratch.  op_step writes every stage temporary and every tile partial before reading it, so
+// nothing in the array has to start at zero, and a call never sees what an earlier one left.
+static __device__ __align__(256) unsigned char scratch[HIS_SWE2D_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_of_current_device(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "his_swe2d_step_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("his_swe2d_step_noalloc: device %d is beyond the %d devices the scratch table holds", dev,
+                  kMaxDevices);
+        return HIS_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "his_swe2d_step_noalloc: hipGetSymbolAddress(scratch)");
+        if (!p) { set_error("his_swe2d_step_noalloc: no device address for the scratch array"); return HIS_E_HIP; }
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int his_swe2d_step_noalloc(const his_grid2d* g, his_state2d st, const double* dt_t, const double* pm,
+                                      double* vols, double* rate, void* stream) {
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!st.h || !st.hu || !st.hv || !st.s_filled || !st.infil_cum || !dt_t || !pm || !vols) {
+        set_error("his_swe2d_step_noalloc: a state, control or output pointer is NULL");
+        return HIS_E_ARG;
+    }
+    const size_t need = ws2d_bytes(*g);
+    if (need > (size_t)HIS_SWE2D_NOALLOC_MAX_BYTES) {
+        set_error("his_swe2d_step_noalloc: this grid needs %zu bytes of scratch, more than the %zu the library "
+                  "holds per GPU; use his_swe2d_step with a workspace", need, (size_t)HIS_SWE2D_NOALLOC_MAX_BYTES);
+        return HIS_E_WORKSPACE;
+    }
+    void* ws = nullptr;
+    rc = scratch_of_current_device(&ws);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     return g->dtype == HIS_F64 ? op_step<double>(g, st, dt_t, pm, vols, rate, ws, s)
                                : op_step<float>(g, st, dt_t, pm, vols, rate, ws, s);

@@ -5,7 +5,8 @@
  * given HIP stream and returns 0 on success or a negative HIS_E_* code (the text
  * of the last error is available from his_last_error()).  No entry point
  * allocates device memory: scratch comes in through a workspace pointer whose
- * size the matching *_workspace_bytes() call returns.
+ * size the matching *_workspace_bytes() call returns, except for
+ * his_swe2d_step_noalloc, whose scratch is a fixed device array of the library.
  *
  * Layouts: 2-D fields are row-major (B, ny, nx) -- rows south to north, columns
  * west to east -- or (ny, nx) when shared by every member (member stride 0).
@@ -72,6 +73,19 @@ size_t his_swe2d_workspace_bytes(const his_grid2d *g);
  *       (SWE2D._max_rate, swe2d.py:179-184, evaluated per member). */
 int his_swe2d_step(const his_grid2d *g, his_state2d st, const double *dt_t, const double *pm, double *vols,
                    double *rate, void *ws, size_t ws_bytes, void *stream);
+
+/* his_swe2d_step with no workspace argument, allocating nothing (no hipMalloc, hipFree or allocating
+ * library call).  No counterpart in hydrointel: it is for callers that cannot allocate device memory
+ * -- code under HIP graph capture, tight serving loops, harnesses that forbid it.  Its scratch is one
+ * fixed device array of HIS_SWE2D_NOALLOC_MAX_BYTES that every GPU the library runs on holds from the
+ * first use until the process ends; a grid whose his_swe2d_workspace_bytes() exceeds it is refused
+ * with HIS_E_WORKSPACE (400x325 grids fit up to 41 members in float32, 19 in float64).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on a GPU (two streams, two host threads) race,
+ * so the caller serialises them.  The stream must belong to the current device.  Same arguments and
+ * results as his_swe2d_step otherwise, and the same kernels. */
+#define HIS_SWE2D_NOALLOC_MAX_BYTES (64u * 1024u * 1024u)
+int his_swe2d_step_noalloc(const his_grid2d *g, his_state2d st, const double *dt_t, const double *pm,
+                           double *vols, double *rate, void *stream);
 
 /* Per-member maximum of (|u|+c)/dx + (|v|+c)/dy over wet cells.
  * Replaces SWE2D._max_rate (hydrointel/solver/swe2d.py:179-184); rate is device (B) float64. */
