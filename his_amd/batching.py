@@ -21,7 +21,7 @@ import time
 import numpy as np
 import torch
 
-from .solver.coupling import CoupledEngine, DeviceEngine, NumericalInstabilityError, N_SUB_LIMIT
+from .solver.coupling import ERROR_CODES, CoupledEngine, DeviceEngine, NumericalInstabilityError
 from . import _lib as L
 
 
@@ -63,7 +63,7 @@ class GroupedEngine:
     def _check_errors(self, t: float, label: str):
         err = self.dev.error.cpu().numpy()
         if err.any():
-            why = {1: f"1-D substeps above {N_SUB_LIMIT}", 2: "non-finite CFL rate"}
+            why = ERROR_CODES
             bad = []
             for gi, m0 in enumerate(self.m0):
                 for b in range(self.engines[gi].two.B):

@@ -650,7 +650,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     F.rate = e->async_fin ? nullptr : e->rate2d;
     F.snap = e->async_fin ? e->snap : nullptr;
     F.volume = e->volume; F.Btot = Btot; F.ngroups = d->ngroups;
-    F.t_out = c.t; F.t_step = e->t_step; F.steps = c.steps;
+    F.t_out = c.t; F.t_step = e->t_step; F.steps = c.steps; F.error = c.error;
     std::vector<DtGroup> dtg;
     for (int gi = 0; gi < d->ngroups; ++gi) {
         const his_group& G = d->group[gi];

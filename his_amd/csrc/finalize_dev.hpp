@@ -5,13 +5,19 @@
 //
 // Each thread sums its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22) and rate maximum
 // (23); each quantity is then reduced by a fixed butterfly within every wave and the waves' results
-// are combined in wave order, so the result is bit-reproducible.
+// are combined in wave order, so the result is bit-reproducible.  A depth or storage sum that is not
+// finite flags the member with error code 3.
 // Volumes follow hydrointel/solver/swe2d.py:_step_core 305-371: boundary volumes from the RK
 // average of the two stages' edge fluxes (324-328), rain (331-337), infiltration and clipping.
 #pragma once
 #include "swe2d_desc.hpp"
 
 namespace his {
+
+// finite test on the bit pattern (exponent not all ones), whatever the unit's floating-point flags
+__device__ __forceinline__ bool finite_bits(double x) {
+    return ((unsigned long long)__double_as_longlong(x) & 0x7ff0000000000000ULL) != 0x7ff0000000000000ULL;
+}
 
 template <int NT>
 __device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int m, double (*red)[24], double* tot) {
@@ -60,6 +66,7 @@ __device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int
     for (int k = 0; k < P_USED - 1; ++k) s2[k] = tot[10 + k];
     const double rmax = tot[23];
     const double area = F.area[gi];
+    if (F.error && F.error[m] == 0 && !(finite_bits(s2[P_VOLH]) && finite_bits(s2[P_VOLS]))) F.error[m] = 3;
     if (F.rate) F.rate[m] = rmax;
     if (F.volume) {
         F.volume[m] = s2[P_VOLH] * area;

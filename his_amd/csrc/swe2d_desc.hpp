@@ -103,6 +103,10 @@ struct FinalizeArgs {
     const double* t_step;
     int32_t* steps;
     int advance[HIS_MAX_GROUPS];
+    // (Btot) or NULL: a member whose depth or storage sum is not finite gets error code 3 (unless it
+    // carries one already).  The float32 kernels are built without NaN semantics, so a NaN cell can
+    // drop out of a max; the float64 sums it enters cannot lose it.
+    int32_t* error;
 };
 int launch_finalize(const FinalizeArgs& f, hipStream_t s);
 

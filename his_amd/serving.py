@@ -40,7 +40,7 @@ from . import workloads as W
 from .config import SolverConfig
 from .engine import Forcing, build_engine_batch
 from .forcing import PiecewiseLinear, sine_tide
-from .solver.coupling import (MAX_PLAUSIBLE_DEPTH_2D_M, MAX_PLAUSIBLE_RISE_1D_M, CoupledEngine)
+from .solver.coupling import (ERROR_CODES, MAX_PLAUSIBLE_DEPTH_2D_M, MAX_PLAUSIBLE_RISE_1D_M, CoupledEngine)
 
 STEP_S = 60.0                     # knot spacing of the per-slot forcing tables
 MAX_STORM_S = 6 * 3600.0
@@ -365,8 +365,7 @@ class Server:
             now = time.perf_counter() - start
             bad = [m for m in self.busy if err[m]]
             for m in bad:
-                self.busy[m][2].status = "unstable: " + ("1-D substeps above limit" if err[m] == 1 else
-                                                         "non-finite CFL rate")
+                self.busy[m][2].status = "unstable: " + ERROR_CODES.get(int(err[m]), f"error {int(err[m])}")
             if bad:
                 self._finish(bad, now, done)
             at_out = [m for m in self.busy if t[m] >= self.t_limit[m] - 1e-9]

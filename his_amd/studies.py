@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -42,7 +43,8 @@ def run_storm(wl: W.Workload, precision: str, t_end: float | None = None, interv
 
     def snap(s):
         nonlocal peaks, when
-        print(f"[{precision}] t = {s.t / 3600:.2f} h, {time.perf_counter() - w_start:.1f} s", flush=True)
+        print(f"[{precision}] t = {s.t / 3600:.2f} h, {time.perf_counter() - w_start:.1f} s", file=sys.stderr,
+              flush=True)
         h = s.h.astype(np.float64)
         if peaks is None:
             peaks = h.copy()
@@ -108,9 +110,11 @@ def main():
     p.add_argument("--members", type=int, default=32)
     p.add_argument("--hours", type=float, default=None)
     p.add_argument("--out", default=None)
+    p.add_argument("--tag", default=None, help="recorded in the output (for example the source revision)")
     a = ap.parse_args()
     if a.cmd == "precision":
         r = precision_study(a.members, a.hours)
+        r["tag"] = a.tag
         txt = json.dumps(r, indent=1)
         if a.out:
             from pathlib import Path

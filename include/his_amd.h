@@ -206,7 +206,9 @@ typedef struct his_control {
                            bnd1d_in, bnd1d_out, clip2d, clip1d, exchanged_in, exchanged_out */
     double *dt_min_seen, *dt_max_seen;   /* (Btot) */
     int32_t *n_sub_max; /* (Btot) */
-    int32_t *error;     /* (Btot) 0 ok; 1 = 1-D substeps above n_sub_limit, 2 = non-finite CFL rate: the
+    int32_t *error;     /* (Btot) 0 ok; 1 = 1-D substeps above n_sub_limit, 2 = non-finite CFL rate,
+                           3 = non-finite 2-D depth or storage (found by the step's float64 sums; a
+                           non-finite momentum reaches the depth through the next step's fluxes): the
                            member stops (like hydrointel's NumericalInstabilityError, coupling.py:169-192) */
     int32_t n_sub_limit;
 } his_control;

@@ -496,3 +496,26 @@ def test_simulate_and_simulate_batch(gpu):
     assert d == {"max_depth_increase_m": 0.0, "area_worsened_ha": 0.0}
     with pytest.raises(ValueError):
         api.simulate(site.replace(manning=site.manning * 0.0), fo1, ter, net, device=gpu)
+
+
+@pytest.mark.parametrize("precision", ["fp32bulk", "fp64"])
+def test_non_finite_depth_stops_the_member(gpu, precision):
+    """A NaN written into one member's depth from the host is found by the step's float64 depth sum:
+    after one step the engine has flagged that member with code 3 (the rate fold alone would not see
+    it: a NaN drops out of a max), and ``run`` raises NumericalInstabilityError naming the member.
+    Every kernel runs normally on NaN values; no index or loop bound depends on the depth."""
+    from his_amd.solver.coupling import NumericalInstabilityError
+    eng, _ = coupled_case(gpu, precision)
+    t = 20.0
+    eng.run(t, out_times=[])
+    de = eng._device()
+    de.refresh()
+    de.t.fill_(t)
+    de.t_end.fill_(1e12)
+    de.t_limit.fill_(1e12)
+    assert de.error.cpu().tolist() == [0, 0]
+    eng.two.h[1, 5, 97] = float("nan")
+    de.step(1)
+    assert de.error.cpu().tolist() == [0, 3]
+    with pytest.raises(NumericalInstabilityError, match="member 1"):
+        eng.run(60.0, t0=t, out_times=[])
