@@ -70,7 +70,7 @@ class GroupedEngine:
                     if err[m0 + b]:
                         bad.append(f"group {gi} member {b}: {why.get(int(err[m0 + b]), err[m0 + b])}")
             raise NumericalInstabilityError(f"unstable state{(' in ' + label) if label else ''} near "
-                                            f"t = {t / 3600:.3f} h: " + ", ".join(bad[:8]) + ". The run is invalid.")
+                                            f"t = {t / 3600:.3f} h: " + ", ".join(bad[:8]) + "; this run cannot be used")
 
     def steps(self) -> list[int]:
         s = self.dev.steps.cpu().numpy()
