@@ -50,9 +50,9 @@ def engine_throughput(wl, steps: int, warmup: int, t_start: float):
     dev = torch.device("cuda")
     eng, fo = build_from_workload(wl, device=dev)
     from his_amd import tuning
+    shape = tuning.tune(eng.two, force=True)
     if t_start > 0:
         eng.run(t_start, out_times=[], progress_s=0)
-    shape = tuning.tune(eng.two, force=True)
     log(f"[bench]   kernel shape {shape} ({tuning.describe(eng.two.dtype, shape)}): "
         + ", ".join(f"{s}:{t:.0f}us" for s, t in tuning.time_shapes(eng.two, 10).items()))
     tuning.tune(eng.two)

@@ -41,7 +41,9 @@ class Net1D(C.Structure):
                 ("m", VP), ("nman", VP), ("dx", VP),
                 ("fb", VP), ("fyb", VP), ("fts", VP), ("fbs", C.c_int64), ("fm", VP),
                 ("fl", VP), ("fr", VP), ("kind", VP), ("g_idx", VP), ("g_src", VP), ("g_sign", VP),
-                ("lf_ptr", VP), ("lf_face", VP), ("rf_ptr", VP), ("rf_face", VP), ("zbank", VP)]
+                ("lf_ptr", VP), ("lf_face", VP), ("rf_ptr", VP), ("rf_face", VP), ("zbank", VP),
+                ("nparts", C.c_int32), ("part_width", C.c_int32), ("p_cptr", VP), ("p_cell", VP), ("p_fptr", VP), ("p_face", VP),
+                ("p_gptr", VP), ("p_ghost", VP)]
 
 
 class Links(C.Structure):

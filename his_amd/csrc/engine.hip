@@ -306,6 +306,10 @@ static int check_net(const his_net1d* n) {
         set_error("network descriptor has a NULL face array");
         return HIS_E_ARG;
     }
+    if (n->nparts > 1 && (!n->p_cptr || !n->p_cell || !n->p_fptr || !n->p_face || !n->p_gptr || !n->p_ghost)) {
+        set_error("network descriptor splits into %d parts but a part array is NULL", n->nparts);
+        return HIS_E_ARG;
+    }
     return 0;
 }
 
@@ -325,7 +329,9 @@ extern "C" int his_swe1d_step(const his_net1d* n, double* A, double* Q, const do
     }
     Net1DDrive d{};
     d.dt = dt; d.pm = pm; d.vols = vols; d.Btot = n->B; d.member0 = 0;
-    return launch_swe1d(*n, A, Q, d, (double*)ws, n->B, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = clear_swe1d_counters(*n, (double*)ws, s))) return rc;
+    return launch_swe1d(*n, A, Q, d, (double*)ws, swe1d_part_width(*n), s);
 }
 
 extern "C" int his_swe1d_max_rate(const his_net1d* n, const double* A, const double* Q, double* rate,
@@ -458,7 +464,8 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
         a.member0 = m0;
         const int k = W.ngroups++;
         W.N[k] = e->net[gi]; W.A[k] = G.A; W.Q[k] = G.Q; W.ws[k] = e->ws1d[gi];
-        W.blk0[k + 1] = W.blk0[k] + G.grid.B;
+        W.width[k] = swe1d_part_width(e->net[gi]);
+        W.blk0[k + 1] = W.blk0[k] + G.grid.B * (e->net[gi].nparts > 1 ? e->net[gi].nparts : 1);
         Net1DDrive& d = W.d[k];
         d.dt = e->d.ctl.dt; d.engine = 1; d.n_sub = e->d.ctl.n_sub; d.t = e->t_step;
         d.t_out = e->d.ctl.t; d.steps = e->d.ctl.steps;
@@ -716,14 +723,23 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
         if (dtype == HIS_F64) e->L64.rate_reset = e->rate2d;
         else e->L32.rate_reset = e->rate2d;
     }
-    // which branch of a step is longer: the 1-D network takes ~70-140 us per step whatever the batch
-    // (one workgroup per member, all members at once); the 2-D stages scale with the member-cells
-    // (~23 G/s in float32, ~4 G/s in float64 on this part)
+    // which branch of a step is longer: a network part's workgroup takes ~15 us plus ~18 us per cell
+    // or face each thread carries (a 4,693-cell network on 1,024 threads: ~100 us), whatever the
+    // batch (all members and parts at once); the 2-D stages scale with the member-cells (~23 G/s in
+    // float32, ~4 G/s in float64 on this part)
     {
-        double cells = 0.0;
-        for (int gi = 0; gi < d->ngroups; ++gi) cells += (double)d->group[gi].grid.B * d->group[gi].grid.ny * d->group[gi].grid.nx;
+        double cells = 0.0, est_1d_us = 0.0;
+        for (int gi = 0; gi < d->ngroups; ++gi) {
+            cells += (double)d->group[gi].grid.B * d->group[gi].grid.ny * d->group[gi].grid.nx;
+            if (!d->group[gi].net) continue;
+            const int w = swe1d_part_width(*d->group[gi].net);
+            int nt = 64;
+            while (nt < w && nt < 1024) nt *= 2;
+            const double us = 15.0 + 18.0 * (double)((w + nt - 1) / nt);
+            est_1d_us = us > est_1d_us ? us : est_1d_us;
+        }
         const double est_2d_us = cells / (dtype == HIS_F64 ? 4.0e3 : 2.3e4);
-        e->one_first = est_2d_us < 100.0;
+        e->one_first = est_2d_us < est_1d_us;
     }
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
@@ -747,6 +763,11 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemcpyAsync(step head)"); }
     he = hipMemsetAsync(e->snap, 0, (size_t)Btot * 3 * 8, s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipMemsetAsync(ledger snapshot)"); }
+    for (int gi = 0; gi < d->ngroups; ++gi) {
+        if (!e->has_net[gi]) continue;
+        const int rc1 = clear_swe1d_counters(e->net[gi], e->ws1d[gi], s);
+        if (rc1) { delete e; return rc1; }
+    }
     he = hipStreamSynchronize(s);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipStreamSynchronize"); }
     if (hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking) != hipSuccess ||
@@ -763,8 +784,11 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
     return 0;
 }
 
+static int engine_follow_shape(his_engine* e);
+
 static int engine_refresh(his_engine* e, const int32_t* mask, hipStream_t s) {
-    int rc;
+    int rc = engine_follow_shape(e);
+    if (rc) return rc;
     if (e->dtype == HIS_F64) rc = launch_state_reduce(e->L64, e->shape, s);
     else rc = launch_state_reduce(e->L32, e->shape, s);
     if (rc) return rc;
@@ -795,6 +819,8 @@ extern "C" int his_engine_refresh_members(his_engine* e, const int32_t* mask, vo
     return engine_refresh(e, mask, (hipStream_t)stream);
 }
 
+static void engine_drop_graphs(his_engine* e);
+
 static int engine_build_graphs(his_engine* e) {
     e->kmulti = 8;
     for (int k = 0; k < 2; ++k) {
@@ -821,9 +847,32 @@ static int engine_build_graphs(his_engine* e) {
     return 0;
 }
 
+// The tile shape is a library-wide setting (his_set_tile_shape, which the autotuner drives).  An
+// engine follows it from its next step or refresh: the tiles, the per-tile partial layout and the
+// finalize's tile ranges are rebuilt and the step graphs captured again.  Between calls no ledger
+// reduction is pending (every run of steps ends with its own), so the layout can change there.
+static int engine_follow_shape(his_engine* e) {
+    int tx, ty;
+    const int shape = shape_of(e->dtype, &tx, &ty);
+    if (shape == e->shape) return 0;
+    engine_drop_graphs(e);
+    if (e->dtype == HIS_F64) tiles_for(e->L64, tx, ty);
+    else tiles_for(e->L32, tx, ty);
+    e->shape = shape;
+    for (int gi = 0; gi < e->d.ngroups; ++gi) {
+        e->F.tile0[gi] = e->dtype == HIS_F64 ? e->L64.g[gi].tile0 : e->L32.g[gi].tile0;
+        e->F.per[gi] = e->dtype == HIS_F64 ? e->L64.g[gi].tiles_x * e->L64.g[gi].tiles_y
+                                           : e->L32.g[gi].tiles_x * e->L32.g[gi].tiles_y;
+    }
+    const hipError_t he = hipMemcpy(e->fin_dev, &e->F, sizeof(FinalizeArgs), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return hip_fail(he, "engine tile shape: hipMemcpy(finalize arguments)");
+    return 0;
+}
+
 extern "C" int his_engine_step(his_engine* e, int32_t n_steps, void* stream) {
     if (!e || n_steps < 0) { set_error("his_engine_step: bad argument"); return HIS_E_ARG; }
     hipStream_t s = (hipStream_t)stream;
+    if (int rc = engine_follow_shape(e)) return rc;
     if (!e->d.use_graph) {
         FinChain chain;
         for (int i = 0; i < n_steps; ++i) {

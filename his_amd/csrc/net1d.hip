@@ -66,19 +66,53 @@ __device__ void rate1d(const his_net1d& N, int b, const double* A, const double*
     }
 }
 
-// All substeps of one coupled step for member b of one network (one workgroup).
+// The cells, faces and ghosts one workgroup steps: part p of the network's split, or all of them.
+struct PartRange {
+    int c0, c1, f0, f1, g0, g1;
+    const int32_t *cell, *face, *ghost;     // NULL = identity (the whole network)
+    __device__ __forceinline__ int c(int i) const { return cell ? cell[i] : i; }
+    __device__ __forceinline__ int f(int i) const { return face ? face[i] : i; }
+    __device__ __forceinline__ int g(int i) const { return ghost ? ghost[i] : i; }
+};
+
+__device__ __forceinline__ PartRange part_range(const his_net1d& N, int p) {
+    PartRange r;
+    if (N.nparts > 1) {
+        r.c0 = N.p_cptr[p]; r.c1 = N.p_cptr[p + 1];
+        r.f0 = N.p_fptr[p]; r.f1 = N.p_fptr[p + 1];
+        r.g0 = N.p_gptr[p]; r.g1 = N.p_gptr[p + 1];
+        r.cell = N.p_cell; r.face = N.p_face; r.ghost = N.p_ghost;
+    } else {
+        r.c0 = 0; r.c1 = N.nc; r.f0 = 0; r.f1 = N.nf; r.g0 = 0; r.g1 = N.ngh;
+        r.cell = r.face = r.ghost = nullptr;
+    }
+    return r;
+}
+
+// All substeps of one coupled step for part p of member b's network (one workgroup).  A split
+// network's parts share no face, so each part's workgroup runs every substep on its own; the
+// member's ledger terms, CFL rate and wet flag are per-part partials that the last part to finish
+// combines in part order (a counter per member, reset by that part), so results are deterministic.
 template <int NT>
 __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double* __restrict__ Qg,
-                             const Net1DDrive& d, double* __restrict__ wsg, const int b, double (*red)[8]) {
+                             const Net1DDrive& d, double* __restrict__ wsg, const int b, const int p,
+                             double (*red)[8]) {
     const int m = d.member0 + b;
     if (d.active && !d.active[m]) return;
     const int nc = N.nc, nf = N.nf;
+    const int P = N.nparts > 1 ? N.nparts : 1;
+    const PartRange R = part_range(N, p);
     double* A = Ag + (int64_t)b * nc;
     double* Q = Qg + (int64_t)b * nc;
     double* ws = wsg + (int64_t)b * (3 * nc + 4 * nf);
     double *y = ws, *u = ws + nc, *th = ws + 2 * nc;
     double *F1 = ws + 3 * nc, *F2 = F1 + nf, *pL = F2 + nf, *pR = pL + nf;
     const double g = N.g, h_dry = N.h_dry;
+    // split networks: this part's ledger partials (b_in, b_out, clip, rate, wet) and the member's counter
+    double* led = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + ((int64_t)b * P + p) * 5;
+    unsigned int* cnt = reinterpret_cast<unsigned int*>(wsg + (int64_t)N.B * (3 * nc + 4 * nf) +
+                                                        (int64_t)N.B * P * 5 + b);
+    double led_in = 0.0, led_out = 0.0, led_clip = 0.0;
     int nsub = 1;
     double dt_full = d.dt[m], t0 = 0.0;
     if (d.engine) {
@@ -86,7 +120,7 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         t0 = d.t[m];
     }
     const double sub = d.engine ? dt_full / nsub : dt_full;
-    if (d.engine && d.ex_vf) {
+    if (d.engine && d.ex_vf && p == 0) {
         // gross exchanged volumes of this step's weir exchange (coupling.py:210-211), fixed order
         const double* vf = d.ex_vf + (int64_t)b * d.ex_stride;
         double part[2] = {0.0, 0.0};
@@ -116,7 +150,8 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         const double dt = sub;
         // ghosts from their interior neighbours (interior cells are not written here)
-        for (int kg = threadIdx.x; kg < N.ngh; kg += NT) {
+        for (int ig = R.g0 + (int)threadIdx.x; ig < R.g1; ig += NT) {
+            const int kg = R.g(ig);
             const int gi = N.g_idx[kg], src = N.g_src[kg];
             const double sg = N.g_sign[kg];
             const Sec s = cell_sec(N, b, gi);
@@ -142,13 +177,15 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             Q[gi] = kd == 3 ? Q_st : (kd == 2 ? Q_in : -Qs);
         }
         __syncthreads();
-        for (int c = threadIdx.x; c < nc; c += NT) {
+        for (int ic = R.c0 + (int)threadIdx.x; ic < R.c1; ic += NT) {
+            const int c = R.c(ic);
             const Sec s = cell_sec(N, b, c);
             y[c] = sec_depth(s, A[c]);
             u[c] = vel1(s, A[c], Q[c], h_dry);
         }
         __syncthreads();
-        for (int f = threadIdx.x; f < nf; f += NT) {
+        for (int jf = R.f0 + (int)threadIdx.x; jf < R.f1; jf += NT) {
+            const int f = R.f(jf);
             const int l = N.fl[f], r = N.fr[f];
             const Sec fs = face_sec(N, b, f);
             const double zL = N.z[(int64_t)b * N.bs + l], zR = N.z[(int64_t)b * N.bs + r];
@@ -179,7 +216,8 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
         // conservative outflow limiter (donor = upwind cell; ghosts unlimited)
-        for (int c = threadIdx.x; c < nc; c += NT) {
+        for (int ic = R.c0 + (int)threadIdx.x; ic < R.c1; ic += NT) {
+            const int c = R.c(ic);
             double out = 0.0;
             for (int p = N.lf_ptr[c]; p < N.lf_ptr[c + 1]; ++p) out += relu(F1[N.lf_face[p]]);
             for (int p = N.rf_ptr[c]; p < N.rf_ptr[c + 1]; ++p) out += relu(-F1[N.rf_face[p]]);
@@ -190,7 +228,8 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
         double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // clip, in(gl), in(gr), out(gl), out(gr)
-        for (int f = threadIdx.x; f < nf; f += NT) {
+        for (int jf = R.f0 + (int)threadIdx.x; jf < R.f1; jf += NT) {
+            const int f = R.f(jf);
             const double f1 = F1[f];
             const double sc = f1 > 0.0 ? th[N.fl[f]] : th[N.fr[f]];
             const double f1s = f1 * sc;
@@ -200,7 +239,8 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             if (gr) { part[2] += relu(-f1s); part[4] += relu(f1s); }
         }
         __syncthreads();
-        for (int c = threadIdx.x; c < nc; c += NT) {
+        for (int ic = R.c0 + (int)threadIdx.x; ic < R.c1; ic += NT) {
+            const int c = R.c(ic);
             const int kd = N.kind[c];
             if (kd >= 2) continue;   // ghosts keep their (refilled) values
             // dA and dQ each summed over the left faces, then the right faces, in list order
@@ -245,7 +285,11 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         block_sum_bcast<NT, 5>(part, red);
         if (threadIdx.x == 0) {
             const double bin = (part[1] + part[2]) * dt, bout = (part[3] + part[4]) * dt;
-            if (d.engine) {
+            if (P > 1) {
+                led_in += bin;
+                led_out += bout;
+                led_clip += part[0];
+            } else if (d.engine) {
                 d.acc[4 * d.Btot + m] += bin;
                 d.acc[5 * d.Btot + m] += bout;
                 d.acc[7 * d.Btot + m] += part[0];
@@ -257,6 +301,7 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
     }
+    double rate_blk = 0.0, wet_blk = 0.0;
     if (want_rate) {
         r_mx = wave_max(r_mx);
         r_wet = wave_max(r_wet);
@@ -264,35 +309,81 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         if (lane == 0) { red[w][0] = r_mx; red[w][1] = r_wet; }
         __syncthreads();
         if (threadIdx.x == 0) {
-            double a = 0.0, bw = 0.0;
             for (int w2 = 0; w2 < NT / 64; ++w2) {
-                a = red[w2][0] > a ? red[w2][0] : a;
-                bw = red[w2][1] > bw ? red[w2][1] : bw;
+                rate_blk = red[w2][0] > rate_blk ? red[w2][0] : rate_blk;
+                wet_blk = red[w2][1] > wet_blk ? red[w2][1] : wet_blk;
             }
-            if (d.rate) d.rate[m] = a;
-            if (d.wet) d.wet[m] = bw > 0.0 ? 1 : 0;
+            if (P == 1) {
+                if (d.rate) d.rate[m] = rate_blk;
+                if (d.wet) d.wet[m] = wet_blk > 0.0 ? 1 : 0;
+            }
         }
     }
-    if (d.engine && d.t_out && threadIdx.x == 0) {
+    if (P > 1 && threadIdx.x == 0) {
+        led[0] = led_in; led[1] = led_out; led[2] = led_clip; led[3] = rate_blk; led[4] = wet_blk;
+        __threadfence();
+        const unsigned int done = atomicAdd(cnt, 1u);
+        if (done == (unsigned int)(P - 1)) {
+            // every other part has published its partials (each fenced before its increment)
+            __threadfence();
+            const volatile double* all = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + (int64_t)b * P * 5;
+            double s_in = 0.0, s_out = 0.0, s_clip = 0.0, s_rate = 0.0, s_wet = 0.0;
+            for (int q = 0; q < P; ++q) {
+                s_in += all[q * 5 + 0];
+                s_out += all[q * 5 + 1];
+                s_clip += all[q * 5 + 2];
+                s_rate = all[q * 5 + 3] > s_rate ? all[q * 5 + 3] : s_rate;
+                s_wet = all[q * 5 + 4] > s_wet ? all[q * 5 + 4] : s_wet;
+            }
+            if (d.engine) {
+                d.acc[4 * d.Btot + m] += s_in;
+                d.acc[5 * d.Btot + m] += s_out;
+                d.acc[7 * d.Btot + m] += s_clip;
+            } else if (d.vols) {
+                d.vols[m] = s_in;
+                d.vols[d.Btot + m] = s_out;
+                d.vols[2 * d.Btot + m] = s_clip;
+            }
+            if (want_rate) {
+                if (d.rate) d.rate[m] = s_rate;
+                if (d.wet) d.wet[m] = s_wet > 0.0 ? 1 : 0;
+            }
+            *cnt = 0u;                 // ready for the next launch
+        }
+    }
+    if (d.engine && d.t_out && p == 0 && threadIdx.x == 0) {
         d.t_out[m] = t0 + dt_full;
         d.steps[m] += 1;
     }
 }
 
+// workgroup k of a network's launch = (member k / parts, part k % parts)
+__device__ __forceinline__ int net_parts(const his_net1d& N) { return N.nparts > 1 ? N.nparts : 1; }
+
 template <int NT>
 __global__ void __launch_bounds__(NT) k_swe1d(const his_net1d N, double* __restrict__ Ag, double* __restrict__ Qg,
                                                const Net1DDrive d, double* __restrict__ wsg) {
     __shared__ double red[NT / 64][8];
-    swe1d_member<NT>(N, Ag, Qg, d, wsg, blockIdx.x, red);
+    const int P = net_parts(N);
+    swe1d_member<NT>(N, Ag, Qg, d, wsg, (int)blockIdx.x / P, (int)blockIdx.x % P, red);
 }
 
-// Every group's network in one launch: workgroups blk0[g] .. blk0[g+1]-1 are group g's members.
+// Every group's network in one launch: workgroups blk0[g] .. blk0[g+1]-1 are group g's (member, part)s.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_swe1d_groups(const Swe1DGroups G) {
     __shared__ double red[NT / 64][8];
     int gi = 0;
     while (gi + 1 < G.ngroups && (int)blockIdx.x >= G.blk0[gi + 1]) ++gi;
-    swe1d_member<NT>(G.N[gi], G.A[gi], G.Q[gi], G.d[gi], G.ws[gi], (int)blockIdx.x - G.blk0[gi], red);
+    const int P = net_parts(G.N[gi]);
+    const int k = (int)blockIdx.x - G.blk0[gi];
+    swe1d_member<NT>(G.N[gi], G.A[gi], G.Q[gi], G.d[gi], G.ws[gi], k / P, k % P, red);
+}
+
+// clears the per-member counters of a split network's workspace (a kernel, not a memset, so a
+// captured graph holds no memset node)
+__global__ void k_clear_u32(unsigned int* p, int n, int stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[(int64_t)i * stride] = 0u;
 }
 
 template <int NT>
@@ -304,15 +395,39 @@ __global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const doub
     rate1d<NT>(N, b, Ag + (int64_t)b * N.nc, Qg + (int64_t)b * N.nc, rate, wet, member0 + b, red);
 }
 
-size_t swe1d_ws_doubles(const his_net1d& n) { return (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf); }
+// per member: y, u, theta (nc each), F1, F2, pL, pR (nf each); then for a split network the parts'
+// ledger partials (5 per part) and one counter slot per member
+size_t swe1d_ws_doubles(const his_net1d& n) {
+    const size_t P = n.nparts > 1 ? (size_t)n.nparts : 1;
+    const size_t split = n.nparts > 1 ? 5 * P + 1 : 0;
+    return (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf + split);
+}
 
-// Workgroup size for a member's network: one thread per cell and per face where the network
-// fits in 1024, in whole waves. Idle waves would still meet every barrier of every substep and
-// hold wave slots the 2-D stages running beside the network could use.
-static int swe1d_threads(const his_net1d& n) {
-    const int w = n.nc > n.nf ? n.nc : n.nf;
+unsigned int* swe1d_counters(const his_net1d& n, double* ws) {
+    return reinterpret_cast<unsigned int*>(ws + (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf) +
+                                           (size_t)n.B * 5 * (size_t)n.nparts);
+}
+
+int clear_swe1d_counters(const his_net1d& n, double* ws, hipStream_t s) {
+    if (n.nparts <= 1) return 0;
+    hipLaunchKernelGGL(k_clear_u32, dim3((n.B + 255) / 256), dim3(256), 0, s, swe1d_counters(n, ws), n.B, 2);
+    HIS_HIP(hipGetLastError());
+    return 0;
+}
+
+// the most cells or faces one workgroup of the network steps (the descriptor's part width when the
+// network is split, which only sizes the workgroup: every loop strides over its part)
+int swe1d_part_width(const his_net1d& n) {
+    if (n.nparts > 1 && n.part_width > 0) return n.part_width;
+    return n.nc > n.nf ? n.nc : n.nf;
+}
+
+// Workgroup size for a network part of `width` cells or faces: one thread per cell and per face
+// where the part fits in 1024, in whole waves.  Idle waves would still meet every barrier of every
+// substep and hold wave slots the 2-D stages running beside the network could use.
+static int swe1d_threads(int width) {
     int nt = 64;
-    while (nt < w && nt < 1024) nt *= 2;
+    while (nt < width && nt < 1024) nt *= 2;
     return nt;
 }
 
@@ -327,11 +442,12 @@ static int with_threads(int nt, F&& launch) {
     }
 }
 
-int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
+int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int width,
                  hipStream_t s) {
-    return with_threads(swe1d_threads(n), [&](auto nt) {
+    const int P = n.nparts > 1 ? n.nparts : 1;
+    return with_threads(swe1d_threads(width), [&](auto nt) {
         constexpr int NT = decltype(nt)::value;
-        hipLaunchKernelGGL(k_swe1d<NT>, dim3(nblocks), dim3(NT), 0, s, n, A, Q, d, ws);
+        hipLaunchKernelGGL(k_swe1d<NT>, dim3(n.B * P), dim3(NT), 0, s, n, A, Q, d, ws);
         HIS_HIP(hipGetLastError());
         return 0;
     });
@@ -340,7 +456,7 @@ int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, 
 int launch_swe1d_groups(const Swe1DGroups& g, hipStream_t s) {
     if (g.ngroups < 1) return 0;
     int nt = 64;
-    for (int gi = 0; gi < g.ngroups; ++gi) nt = swe1d_threads(g.N[gi]) > nt ? swe1d_threads(g.N[gi]) : nt;
+    for (int gi = 0; gi < g.ngroups; ++gi) nt = swe1d_threads(g.width[gi]) > nt ? swe1d_threads(g.width[gi]) : nt;
     return with_threads(nt, [&](auto t) {
         constexpr int NT = decltype(t)::value;
         hipLaunchKernelGGL(k_swe1d_groups<NT>, dim3(g.blk0[g.ngroups]), dim3(NT), 0, s, g);

@@ -29,8 +29,12 @@ struct Net1DDrive {
     int32_t *steps;
 };
 
-int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int nblocks,
+// one workgroup per (member, part); width = the most cells or faces of any part (swe1d_part_width)
+int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int width,
                  hipStream_t s);
+int swe1d_part_width(const his_net1d& n);
+// a split network's workspace holds one counter per member that must be zero before its first step
+int clear_swe1d_counters(const his_net1d& n, double* ws, hipStream_t s);
 int launch_swe1d_rate(const his_net1d& n, const double* A, const double* Q, double* rate, int32_t* wet,
                       int member0, hipStream_t s, const int32_t* only);   // only: NULL or (Btot) member mask
 
@@ -76,10 +80,12 @@ struct ExGroups {
 };
 int launch_exchange_groups(ExGroups& g, hipStream_t s);
 
-// Several groups' 1-D networks in one launch: workgroups blk0[g] .. blk0[g+1]-1 run group g.
+// Several groups' 1-D networks in one launch: workgroups blk0[g] .. blk0[g+1]-1 run group g, one per
+// (member, part); width[g] = the most cells or faces of any of group g's parts.
 struct Swe1DGroups {
     int ngroups;
     int blk0[HIS_MAX_GROUPS + 1];
+    int width[HIS_MAX_GROUPS];
     his_net1d N[HIS_MAX_GROUPS];
     double *A[HIS_MAX_GROUPS], *Q[HIS_MAX_GROUPS], *ws[HIS_MAX_GROUPS];
     Net1DDrive d[HIS_MAX_GROUPS];

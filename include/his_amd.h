@@ -120,13 +120,22 @@ typedef struct his_net1d {
      * lf_face[lf_ptr[c] .. lf_ptr[c+1]), faces with fr == c are rf_face[rf_ptr[c] .. rf_ptr[c+1]) */
     const int32_t *lf_ptr, *lf_face, *rf_ptr, *rf_face;
     const double *zbank;                   /* (nc) shared bank crest, for the plausibility guard */
+    /* Optional split of the network into nparts parts that no face joins (unions of whole connected
+     * components), which separate workgroups then step without synchronising: part p owns the cells
+     * p_cell[p_cptr[p] .. p_cptr[p+1]), the faces p_face[p_fptr[p] .. p_fptr[p+1]) and the ghosts
+     * p_ghost[p_gptr[p] .. p_gptr[p+1]) (indices into g_idx), each list ascending.  nparts <= 1 =
+     * one part, the whole network (the arrays may then be NULL). */
+    int32_t nparts;
+    int32_t part_width;                    /* the most cells or faces of any part: sizes the workgroup */
+    const int32_t *p_cptr, *p_cell, *p_fptr, *p_face, *p_gptr, *p_ghost;
 } his_net1d;
 
 size_t his_swe1d_workspace_bytes(const his_net1d *n);
 
 /* One 1-D step of every member.  Replaces SWE1D._step_core (hydrointel/solver/swe1d.py:320-368).
  * A, Q: device (B, nc) updated in place.  dt: device (B).  pm: device (2, B) = q_in, eta_sea.
- * vols: device (3, B) out = b_in, b_out, clip (Step1DVolumes, swe1d.py:169-173). */
+ * vols: device (3, B) out = b_in, b_out, clip (Step1DVolumes, swe1d.py:169-173).
+ * One workgroup per member and part; the cell and face arithmetic does not depend on the split. */
 int his_swe1d_step(const his_net1d *n, double *A, double *Q, const double *dt, const double *pm, double *vols,
                    void *ws, size_t ws_bytes, void *stream);
 

@@ -519,3 +519,29 @@ def test_non_finite_depth_stops_the_member(gpu, precision):
     assert de.error.cpu().tolist() == [0, 3]
     with pytest.raises(NumericalInstabilityError, match="member 1"):
         eng.run(60.0, t0=t, out_times=[])
+
+
+def test_engine_follows_a_tile_shape_change(gpu):
+    """The tile shape is a library-wide setting; an engine that has captured its step graphs with one
+    shape takes the new one from its next step (new tiles, partial layout and ledger ranges), so the
+    state and the ledger match an engine that used the new shape throughout."""
+    from his_amd import _lib
+    try:
+        _lib.set_tile_shape(torch.float64, 0)
+        eng_a, _ = coupled_case(gpu)
+        eng_a.run(12.0, out_times=[])
+        _lib.set_tile_shape(torch.float64, 6)
+        eng_a.run(30.0, t0=12.0, out_times=[])
+        eng_b, _ = coupled_case(gpu)
+        eng_b.run(12.0, out_times=[])
+        eng_b.run(30.0, t0=12.0, out_times=[])
+    finally:
+        _lib.set_tile_shape(torch.float64, 0)
+    assert eng_a.log.steps == eng_b.log.steps
+    for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
+        assert_rel(_f64(getattr(eng_a.two, name)), _f64(getattr(eng_b.two, name)), 1e-12, 1e-14, name)
+    sa, sb = eng_a.mb.sources(), eng_b.mb.sources()
+    for k in sa:
+        assert_rel(np.asarray(sa[k]), np.asarray(sb[k]), 1e-12, 1e-9, f"ledger {k}")
+    err = np.asarray(eng_a.mb.check(eng_a.total_volume(), 30.0))
+    assert err.max() < 1e-12
