@@ -1,20 +1,27 @@
-"""``simulate`` / ``simulate_batch``: one storm, or B storms over one terrain, to FloodResults.
+"""The run-level interface: ``simulate`` and ``simulate_batch`` with hydrointel's signatures.
 
-Mirrors hydrointel/api.py:211-324 for the part on the engine's path. Each call:
-1. checks the site arrays;
-2. builds the coupled engine (``build_engine`` / ``build_engine_batch``);
-3. runs it to the end of the storm, stopping at every output time;
-4. assembles ``FloodResult``:
-   * ``depth_max`` is the maximum over the output-time snapshots (api.py:239), not over
-     every step, and ``speed_max`` likewise from the snapshot velocities;
-   * channel series cover the interior 1-D cells (kind <= 1);
-   * infiltrated and stored volumes and the mass-balance error come per member.
+    simulate(site, scenario, output_interval_s=None, compare_to_baseline=False) -> FloodResult
+    simulate_batch(sites, scenarios, output_interval_s=None) -> [FloodResult]
 
-hydrointel turns a ``Scenario`` (return period, RCP, SSP, horizon) into rainfall, tide and
-inflow through its forcing package and land-use tables. Those sit upstream of the engine
-and are not part of this package, so these functions take the storm's ``Forcing`` and the
-terrain directly. Everything downstream of that matches. So does the B=1 routing: a
-batch of one is the plain solver, as in api.py:274-276.
+as hydrointel/api.py:211-324 declares them, so ``generate``, the benchmark stages and an optimiser
+call them unchanged.  Like hydrointel, they work on a process-wide context (``configure``): the
+solver configuration, the device and the ``Domain`` (terrain, channel network, land use).  Each call
+
+1. validates the scenario and the site (ValueError for shapes, non-finite values and ranges);
+2. turns each scenario into rain, tide and inflow through the registered resolver
+   (``his_amd.scenarios``) and the scenario's land use into Horton fields scaled by the site's
+   infiltration multiplier;
+3. builds the coupled engine (``build_engine`` / ``build_engine_batch``), runs it to the end of the
+   storm stopping at every output time, and asserts mass balance per member (AssertionError);
+4. assembles ``FloodResult``: ``depth_max`` is the maximum over the output-time snapshots
+   (hydrointel/api.py:239) and ``speed_max`` likewise; channel series cover the interior 1-D cells;
+   volumes and the mass-balance error are per member; ``extras`` carries the forcing, the run log,
+   the ledger and mass series, the boundary volumes and the exposure metrics.
+
+A batch of one is routed to ``simulate`` (api.py:274-276).  ``compare_to_baseline`` fills
+``max_depth_increase_m`` and ``area_worsened_ha`` against a run of ``baseline_site(scenario)``,
+cached per scenario and output interval.  ``predict`` (the neural surrogate) is not part of this
+package.
 """
 from __future__ import annotations
 
@@ -25,8 +32,9 @@ import numpy as np
 import torch
 
 from .config import SolverConfig
-from .engine import Forcing, build_engine_batch
-from .workloads import MMH
+from .domain import landuse as LU
+from .engine import build_engine, build_engine_batch
+from .scenarios import Scenario, label, population_factor, resolve, validate
 
 WORSENED_THRESHOLD_M = 0.01
 
@@ -65,12 +73,86 @@ class FloodResult:
     area_worsened_ha: float | None = None
 
 
-def baseline_site(terrain, network, manning: float = 0.035, storage_m: float = 0.005) -> SiteState:
-    """No interventions: uniform roughness and retention on land, kappa = 1, gamma = 1."""
-    land = ~terrain.sea
-    t = lambda a: torch.as_tensor(np.asarray(a, np.float32))
-    return SiteState(t(np.where(land, storage_m, 0.0)), t(np.ones(terrain.shape)),
-                     t(np.full(terrain.shape, manning)), t(np.ones(network.n_reaches)))
+# ---------------------------------------------------------------------------- context
+class Context:
+    """What every call shares: solver configuration, device and domain; land use per SSP and year."""
+
+    def __init__(self, domain=None, cfg: SolverConfig | None = None, device=None):
+        self.cfg = cfg or SolverConfig()
+        self._domain = domain
+        self.device = device
+        self._landuse: dict = {}
+
+    @property
+    def domain(self):
+        if self._domain is None:
+            self._domain = default_domain()
+        return self._domain
+
+    def landuse(self, ssp: str, year: int) -> np.ndarray:
+        key = (ssp, int(year))
+        if key not in self._landuse:
+            self._landuse[key] = grown_landuse(self.domain, ssp, int(year))
+        return self._landuse[key]
+
+    def provenance(self, engine: str = "engine") -> dict:
+        return dict(engine=engine, backend="his_amd (gfx950 HIP)", precision=self.cfg.precision,
+                    device=str(self.device or "cuda"), domain=getattr(self.domain, "name", ""))
+
+
+_CTX: list = [None]
+_BASELINE_PEAK: dict = {}
+
+
+def default_domain():
+    """The 8 km x 6.5 km coastal plain at 20 m (400 x 325) with the 60-reach network and the seeded
+    land-use map."""
+    from . import workloads as W
+    from .domain.site import domain_from
+    ter = W.make_terrain(400, 325, 20.0)
+    return domain_from(ter, W.network60(ter), name="coastal plain 400x325@20m")
+
+
+def configure(domain=None, cfg: SolverConfig | None = None, device=None) -> Context:
+    _CTX[0] = Context(domain, cfg, device)
+    _BASELINE_PEAK.clear()
+    return _CTX[0]
+
+
+def context() -> Context:
+    return _CTX[0] or configure()
+
+
+def grown_landuse(domain, ssp: str, year: int) -> np.ndarray:
+    """The domain's land use at a horizon year under an SSP: cropland, bare ground and grass turn
+    residential (dense urban nearest the core) at the pathway's expansion rate per decade since 2020,
+    nearest the existing town first.  Deterministic."""
+    from .scenarios import SSP_TABLE
+    base = np.asarray(domain.landuse)
+    rate = SSP_TABLE[ssp][0]
+    grow = np.isin(base, (LU.CROPLAND, LU.BARE, LU.GRASS)) & ~domain.channel & ~domain.sea
+    n_new = int(round(rate * max(year - 2020, 0) / 10.0 * grow.sum()))
+    if n_new <= 0:
+        return base.copy()
+    from scipy.ndimage import distance_transform_edt
+    town = np.isin(base, (LU.DENSE_URBAN, LU.RESIDENTIAL, LU.ROAD))
+    dist = distance_transform_edt(~town) if town.any() else np.zeros(base.shape)
+    order = np.argsort(np.where(grow, dist, np.inf), axis=None, kind="stable")[:n_new]
+    out = base.copy().reshape(-1)
+    near = dist.reshape(-1)[order] <= np.percentile(dist.reshape(-1)[order], 25)
+    out[order] = np.where(near, LU.DENSE_URBAN, LU.RESIDENTIAL)
+    return out.reshape(base.shape)
+
+
+# ------------------------------------------------------------------------- sites
+def baseline_site(scenario: Scenario) -> SiteState:
+    """No interventions on the scenario's land surface: the class table's retention and roughness,
+    kappa = 1 and gamma = 1 (hydrointel/api.py:175-184)."""
+    ctx = context()
+    lu = ctx.landuse(scenario.ssp, scenario.horizon_year)
+    f32 = lambda a: torch.as_tensor(np.asarray(a, np.float32))
+    return SiteState(f32(LU.lookup(lu, "storage_m")), f32(np.ones(ctx.domain.shape)), f32(LU.lookup(lu, "manning")),
+                     f32(np.ones(ctx.domain.network.n_reaches)))
 
 
 def _np(a) -> np.ndarray:
@@ -79,129 +161,167 @@ def _np(a) -> np.ndarray:
     return np.asarray(a, dtype=np.float64)
 
 
-def validate_site(site: SiteState, terrain, network) -> None:
-    """hydrointel/api.py:461-478: shapes, finiteness and ranges, as ValueError."""
+def validate_site(site: SiteState, domain) -> None:
+    """Shapes, finiteness and ranges of a site (hydrointel/api.py:456-473), as ValueError."""
     for name in ("storage_depth", "infil_multiplier", "manning"):
-        a = getattr(site, name)
-        if tuple(a.shape) != terrain.shape:
-            raise ValueError(f"SiteState.{name} has shape {tuple(a.shape)}, domain grid is {terrain.shape} (ny, nx)")
-        if not torch.isfinite(torch.as_tensor(a)).all():
-            raise ValueError(f"SiteState.{name} contains non-finite values")
-    if tuple(site.channel_gamma.shape) != (network.n_reaches,):
-        raise ValueError(f"SiteState.channel_gamma must have shape ({network.n_reaches},)")
-    if float(site.storage_depth.min()) < 0:
-        raise ValueError("SiteState.storage_depth must be >= 0 m")
-    if float(site.infil_multiplier.min()) < 1.0 - 1e-6:
-        raise ValueError("SiteState.infil_multiplier must be >= 1")
-    if float(site.manning.min()) <= 0:
-        raise ValueError("SiteState.manning must be > 0")
-    if float(site.channel_gamma.min()) <= 0:
-        raise ValueError("SiteState.channel_gamma must be > 0")
+        a = _np(getattr(site, name))
+        if a.shape != tuple(domain.shape):
+            raise ValueError(f"SiteState.{name} is {a.shape} but the domain grid is {tuple(domain.shape)} (ny, nx)")
+        if not np.isfinite(a).all():
+            raise ValueError(f"SiteState.{name} holds non-finite values")
+    n_reaches = domain.network.n_reaches
+    if _np(site.channel_gamma).shape != (n_reaches,):
+        raise ValueError(f"SiteState.channel_gamma needs one value per reach ({n_reaches},)")
+    checks = (("storage_depth", lambda a: a.min() >= 0, "must be >= 0 m"),
+              ("infil_multiplier", lambda a: a.min() >= 1.0 - 1e-6, "must be >= 1"),
+              ("manning", lambda a: a.min() > 0, "must be > 0"),
+              ("channel_gamma", lambda a: a.min() > 0, "must be > 0"))
+    for name, ok, why in checks:
+        if not ok(_np(getattr(site, name))):
+            raise ValueError(f"SiteState.{name} {why}")
 
 
-def _horton_fields(terrain, kappa):
-    """Horton f0 = 60, fc = 10 mm/h and k = 2 /h on land, infiltration capacity scaled by kappa."""
-    land = ~terrain.sea
-    return (np.where(land, 60.0 * MMH, 0.0) * kappa, np.where(land, 10.0 * MMH, 0.0) * kappa,
-            np.where(land, 2.0 / 3600.0, 0.0))
+def check_site(site: SiteState, scenario: Scenario | None = None):
+    """(in distribution, warnings).  The site is validated; a training envelope belongs to the
+    surrogate, which is not part of this package, so no site is vouched for."""
+    validate_site(site, context().domain)
+    return False, ["no training envelope here (the surrogate is not part of this package): the engine result "
+                   "is the ground truth"]
 
 
-def _run(sites, forcing: Forcing, terrain, network, output_interval_s, cfg, device, label):
-    f0, fc, k = zip(*(_horton_fields(terrain, _np(s.infil_multiplier)) for s in sites))
-    eng = build_engine_batch(cfg, terrain, network, forcing, [_np(s.manning) for s in sites],
-                             [_np(s.storage_depth) for s in sites], list(f0), list(fc), list(k),
-                             [_np(s.channel_gamma) for s in sites], device=device, batched=len(sites) > 1,
-                             horton_t0=forcing.storm_start)
+# -------------------------------------------------------------------------- runs
+def _horton(lu, kappa):
+    return LU.horton_fields(lu, kappa)
+
+
+def _snapshots(eng, t_end: float, times, label_: str):
     snaps = []
-    times = forcing.times(output_interval_s or cfg.output_interval_s)
-    err = eng.run(forcing.t_end, out_times=times, on_snapshot=snaps.append, label=label)
-    return eng, snaps, err
+    err = eng.run(t_end, out_times=times, on_snapshot=snaps.append, label=label_)
+    return snaps, err
 
 
 def _member(v, b: int, batched: bool):
-    """Member b's value of a ledger or mass-series entry (arrays over the batch when batched)."""
     if batched and getattr(v, "ndim", 0):
         return float(np.asarray(v, dtype=np.float64).reshape(-1)[b])
     return float(v) if isinstance(v, (int, float, np.floating, np.integer)) else v
 
 
-def _result(eng, snaps, err, b, batched: bool, prov: dict, extras: dict) -> FloodResult:
+def _result(eng, snaps, err, b: int, batched: bool, prov: dict, ok_warn, scenario, extras: dict) -> FloodResult:
     interior = torch.as_tensor(eng.one.topo.kind <= 1)
     pick = (lambda a: a[b]) if batched else (lambda a: a)
     st = lambda k: torch.as_tensor(np.stack([pick(getattr(s, k)) for s in snaps]))
     h, u, v = st("h"), st("u"), st("v")
     src = eng.mb.sources()
-    s_b = {k: float(np.asarray(v_, dtype=np.float64).reshape(-1)[b]) for k, v_ in src.items()}
-    sv = eng.two.storage_volume().detach().cpu().numpy()
+    s_b = {k: float(np.asarray(x, dtype=np.float64).reshape(-1)[b]) for k, x in src.items()}
+    stored = np.asarray(eng.two.storage_volume().detach().cpu().numpy()).reshape(-1)[b]
+    depth_max = h.amax(0)
+    log = eng.log
+    if batched:
+        log = dataclasses.replace(log, exchanged_in_m3=_member(log.exchanged_in_m3, b, True),
+                                  exchanged_out_m3=_member(log.exchanged_out_m3, b, True), series=[])
     return FloodResult(
-        depth_max=h.amax(0), depth_series=h, u=u, v=v,
+        depth_max=depth_max, depth_series=h, u=u, v=v,
         speed_max=torch.sqrt(u.double() ** 2 + v.double() ** 2).amax(0).float(),
         channel_eta=st("eta1")[:, interior], channel_q=st("q1")[:, interior],
         times=torch.as_tensor(np.array([x.t for x in snaps])),
-        infiltrated_volume_m3=s_b["infiltration"], stored_volume_m3=float(np.asarray(sv).reshape(-1)[b]),
+        infiltrated_volume_m3=s_b["infiltration"], stored_volume_m3=float(stored),
         mass_balance_error=float(np.asarray(err, dtype=np.float64).reshape(-1)[b]), provenance=prov,
-        extras={"run_log": eng.log, "sources": s_b, "channel_cells": np.nonzero(eng.one.topo.kind <= 1)[0],
+        in_distribution=ok_warn[0], distribution_warnings=list(ok_warn[1]),
+        extras={"run_log": log, "sources": s_b, "channel_cells": np.nonzero(eng.one.topo.kind <= 1)[0],
                 "topology": eng.one.topo, "boundary_in_m3": s_b["bnd2d_in"] + s_b["bnd1d_in"],
                 "boundary_out_m3": s_b["bnd2d_out"] + s_b["bnd1d_out"],
                 # volume ledger at each output time and the mass-balance checks, this member's
                 # (hydrointel/api.py:245-247, 315-320)
-                "ledger": [{k: _member(v, b, batched) for k, v in x.ledger.items()} for x in snaps],
-                "mass_series": [{k: _member(v, b, batched) for k, v in rec.items()} for rec in eng.mb.series],
-                **extras})
+                "ledger": [{k: _member(x, b, batched) for k, x in s.ledger.items()} for s in snaps],
+                "mass_series": [{k: _member(x, b, batched) for k, x in r.items()} for r in eng.mb.series],
+                **extras, **exposure_metrics(depth_max, scenario)})
 
 
-def simulate(site: SiteState, forcing: Forcing, terrain, network, output_interval_s: float | None = None,
-             cfg: SolverConfig | None = None, device=None, baseline_depth_max=None) -> FloodResult:
-    """One storm through the coupled engine; raises AssertionError if mass balance fails.
-    ``baseline_depth_max`` fills max_depth_increase_m / area_worsened_ha (api.py:411-427)."""
-    cfg = cfg or SolverConfig()
-    validate_site(site, terrain, network)
-    if forcing.rain.n != 1:
-        raise ValueError(f"simulate takes one storm; the forcing holds {forcing.rain.n}")
-    eng, snaps, err = _run([site], forcing, terrain, network, output_interval_s, cfg, device, "simulate")
-    prov = dict(engine="his_amd", precision=cfg.precision, device=str(eng.two.device))
-    res = _result(eng, snaps, err, 0, False, prov, {"forcing": forcing})
-    if baseline_depth_max is not None:
-        res = dataclasses.replace(res, **disbenefit(res.depth_max, baseline_depth_max, terrain, network))
+def simulate(site: SiteState, scenario: Scenario, output_interval_s: float | None = None,
+             compare_to_baseline: bool = False) -> FloodResult:
+    """One storm through the coupled engine; raises AssertionError if mass balance fails."""
+    ctx = context()
+    validate(scenario)
+    ok_warn = check_site(site, scenario)
+    fo = resolve(scenario, ctx.domain)
+    lu = ctx.landuse(scenario.ssp, scenario.horizon_year)
+    eng = build_engine(ctx.cfg, ctx.domain, fo, _np(site.storage_depth), _np(site.infil_multiplier),
+                       _np(site.manning), _np(site.channel_gamma), landuse=lu, device=ctx.device)
+    times = fo.times(output_interval_s or ctx.cfg.output_interval_s)
+    snaps, err = _snapshots(eng, fo.t_end, times, label(scenario))
+    res = _result(eng, snaps, err, 0, False, ctx.provenance(), ok_warn, scenario, {"forcing": fo})
+    if compare_to_baseline:
+        ref = _baseline_peak(scenario, output_interval_s)
+        res = dataclasses.replace(res, **disbenefit(res.depth_max, ref))
     return res
 
 
-def simulate_batch(sites: list, forcing: Forcing, terrain, network, output_interval_s: float | None = None,
-                   cfg: SolverConfig | None = None, device=None) -> list:
-    """B storms over one terrain on one set of kernels (shared dt, as hydrointel/api.py:272-324);
-    per-member forcing rows, per-member mass balance; a batch of one is ``simulate``."""
-    cfg = cfg or SolverConfig()
+def simulate_batch(sites: list, scenarios: list, output_interval_s: float | None = None) -> list:
+    """B storms on one set of kernels: shared terrain, time base and time step (the smallest any
+    member needs); per-member site, return period, pathway, tide and mass balance."""
+    ctx = context()
+    if len(sites) != len(scenarios):
+        raise ValueError(f"{len(sites)} sites but {len(scenarios)} scenarios")
     if not sites:
         return []
-    if forcing.rain.n != len(sites):
-        raise ValueError(f"{len(sites)} sites but {forcing.rain.n} forcing members")
     if len(sites) == 1:
-        return [simulate(sites[0], forcing, terrain, network, output_interval_s, cfg, device)]
-    for s in sites:
-        validate_site(s, terrain, network)
-    eng, snaps, err = _run(sites, forcing, terrain, network, output_interval_s, cfg, device,
-                           f"batch of {len(sites)}")
-    prov = dict(engine="his_amd", precision=cfg.precision, device=str(eng.two.device))
-    return [_result(eng, snaps, err, b, True, prov, {"forcing": forcing, "batch_index": b, "batch_size": len(sites)})
-            for b in range(len(sites))]
+        return [simulate(sites[0], scenarios[0], output_interval_s)]
+    checks = []
+    for s, sc in zip(sites, scenarios):
+        validate(sc)
+        checks.append(check_site(s, sc))
+    fos = [resolve(sc, ctx.domain) for sc in scenarios]
+    lus = [ctx.landuse(sc.ssp, sc.horizon_year) for sc in scenarios]
+    eng, fo = build_engine_batch(ctx.cfg, ctx.domain, fos, [_np(s.storage_depth) for s in sites],
+                                 [_np(s.infil_multiplier) for s in sites], [_np(s.manning) for s in sites],
+                                 [_np(s.channel_gamma) for s in sites], lus, device=ctx.device)
+    times = fo.times(output_interval_s or ctx.cfg.output_interval_s)
+    snaps, err = _snapshots(eng, fo.t_end, times,
+                            f"batch of {len(sites)}: " + "; ".join(label(sc) for sc in scenarios[:4]))
+    prov = ctx.provenance()
+    return [_result(eng, snaps, err, b, True, prov, checks[b], sc,
+                    {"forcing": fos[b], "batch_index": b, "batch_size": len(sites)})
+            for b, sc in enumerate(scenarios)]
 
 
-def land_mask(terrain, network) -> np.ndarray:
-    """Land = not sea and not under a channel reach (a 1-D cell's 2-D cell)."""
-    from .solver.swe1d import build_topology
-    tp = build_topology(network)
-    ch = np.zeros(terrain.shape, bool)
-    i = np.clip((tp.x / terrain.dx).astype(int), 0, terrain.nx - 1)
-    j = np.clip((tp.y / terrain.dx).astype(int), 0, terrain.ny - 1)
-    ch[j[tp.kind <= 1], i[tp.kind <= 1]] = True
-    return ~(terrain.sea | ch)
+def _baseline_peak(scenario: Scenario, output_interval_s=None) -> torch.Tensor:
+    key = (scenario, float(output_interval_s or context().cfg.output_interval_s))
+    if key not in _BASELINE_PEAK:
+        _BASELINE_PEAK[key] = simulate(baseline_site(scenario), scenario, output_interval_s).depth_max
+    return _BASELINE_PEAK[key]
 
 
-def disbenefit(depth_max, baseline_depth_max, terrain, network) -> dict:
-    """Largest rise of peak depth on land, and land area where it rose by more than 1 cm
-    (hydrointel/api.py:411-427)."""
-    land = torch.as_tensor(land_mask(terrain, network))
+def baseline(scenario: Scenario, engine: str = "simulate") -> FloodResult:
+    """The flood with no interventions.  Only the engine is part of this package."""
+    if engine != "simulate":
+        raise ValueError("engine must be 'simulate' (the surrogate 'predict' is not part of this package)")
+    return simulate(baseline_site(scenario), scenario)
+
+
+# ------------------------------------------------------------------------ metrics
+def land_mask(domain=None) -> np.ndarray:
+    """Land = neither sea nor channel footprint."""
+    return (domain or context().domain).land
+
+
+def disbenefit(depth_max, baseline_depth_max) -> dict:
+    """Largest rise of peak depth on land (0 if nowhere rises), and the land area where it rose by more
+    than 1 cm (hydrointel/api.py:400-418)."""
+    dom = context().domain
+    land = torch.as_tensor(land_mask(dom))
     d = depth_max.detach().float().cpu() - torch.as_tensor(baseline_depth_max).detach().float().cpu()
     rise = torch.clamp(d[land].max(), min=0.0)
     return {"max_depth_increase_m": float(rise),
-            "area_worsened_ha": float((d[land] > WORSENED_THRESHOLD_M).sum()) * terrain.dx ** 2 / 1e4}
+            "area_worsened_ha": float((d[land] > WORSENED_THRESHOLD_M).sum()) * dom.dx ** 2 / 1e4}
+
+
+def exposure_metrics(depth_max, scenario: Scenario, threshold_m: float = 0.15) -> dict:
+    """The exposure raster (class exposure index times the SSP's population factor) and the
+    exposure-weighted land area flooded deeper than ``threshold_m`` (hydrointel/api.py:430-442)."""
+    ctx = context()
+    dom = ctx.domain
+    lu = ctx.landuse(scenario.ssp, scenario.horizon_year)
+    expo = LU.lookup(lu, "exposure") * population_factor(scenario.ssp)
+    wet = depth_max.detach().cpu().numpy() >= threshold_m
+    return {"exposure": expo.astype(np.float32),
+            f"exposure_weighted_area_gt_{threshold_m:g}m_km2": float(np.sum(expo[wet & dom.land]) * dom.dx ** 2 / 1e6)}

@@ -146,3 +146,28 @@ def select(series: Series, idx) -> Series:
         out.values = series.values[idx].copy()
     out.n = idx.size
     return out
+
+
+def stack(members: list) -> Series:
+    """One batched series from one-member series of the same form on a shared time base (the same
+    knots, or the same tidal constituents): hydrointel's BatchForcing enforces a shared time base too
+    (hydrointel/engine.py:104-143).  ValueError when the members do not share one."""
+    first = members[0]
+    kind = type(first)
+    if any(type(m) is not kind for m in members):
+        raise ValueError("batched forcing members mix series forms")
+    if isinstance(first, (PiecewiseLinear, PiecewiseConstant)):
+        if any(m.t0 != first.t0 or m.step != first.step or m.values.shape[1] != first.values.shape[1]
+               for m in members):
+            raise ValueError("batched forcing members do not share a time base (t0, step, knots)")
+        return kind(first.t0, first.step, np.concatenate([m.values for m in members], axis=0))
+    if isinstance(first, Harmonic):
+        if any(not (np.array_equal(m.amp, first.amp) and np.array_equal(m.w, first.w)
+                    and np.array_equal(m.phase, first.phase)) for m in members):
+            raise ValueError("batched tide members do not share their constituents")
+        out = Harmonic(first.amp, [2 * np.pi / w for w in first.w], first.phase,
+                       np.concatenate([m.base for m in members]), np.concatenate([m.shift for m in members]),
+                       np.concatenate([m.on for m in members]))
+        out.w = first.w.copy()
+        return out
+    return Constant(np.concatenate([np.atleast_1d(m.values) for m in members]))

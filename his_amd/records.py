@@ -136,7 +136,8 @@ def simulation_record(res, f: int, dx: float, *, site, meta: dict | None = None,
     when given; ``forcing_factors`` (for instance rain_factor, slr_m, arf) join the forcing
     block."""
     fo = res.extras["forcing"]
-    b = int(res.extras.get("batch_index", 0))
+    # a result's forcing holds its own storm (one member) or the whole batch's
+    b = 0 if getattr(fo.rain, "n", 1) == 1 else int(res.extras.get("batch_index", 0))
     times = res.times.numpy()
     h, u, v = res.depth_series.numpy(), res.u.numpy(), res.v.numpy()
     hc = coarsen_mean(h, f)
@@ -188,45 +189,71 @@ def index_entry(path: Path, res, sea: np.ndarray, meta: dict, f: int, wall_s: fl
             "in_distribution": bool(res.in_distribution)}
 
 
-def write_storms(out, storms: list, forcing, terrain, network, f: int = 1, compress: bool = False,
-                 cfg=None, device=None, forcing_steps: int = 96, output_interval_s: float | None = None) -> dict:
-    """Run ``storms`` -- a list of (sim_id, SiteState, meta dict) -- as one batch with per-member
-    ``forcing`` and write each storm's record and index entry (the loop body of generate.py:
-    229-321). Resumable: a storm whose record exists and which the index holds is skipped, and
-    so is one recorded as failed. A batch of one that the engine cannot simulate is recorded as
-    failed; in a larger batch the failing member cannot be told apart and the error propagates."""
+def horton_rate_at(landuse, kappa, times, f: int) -> np.ndarray:
+    """Horton capacity f(t) = fc + (f0 - fc) exp(-k t) [mm/h] of the land use scaled by kappa, at each
+    output time, block-averaged onto the storage grid: (nt, ny/f, nx/f)."""
+    from .domain import landuse as LU
+    f0, fc, k = LU.horton_fields(landuse, kappa)
+    out = [coarsen_mean((fc + (f0 - fc) * np.exp(-k * float(t))) / MMH, f) for t in np.asarray(times)]
+    return np.stack(out).astype(np.float32)
+
+
+def landuse_fractions(landuse) -> np.ndarray:
+    """One-hot class map (n_classes, ny, nx); ``simulation_record`` block-averages it into each class's
+    share of every storage-grid cell."""
+    from .domain import landuse as LU
+    lu = np.asarray(landuse)
+    return np.stack([(lu == c).astype(np.float64) for c in range(LU.N_CLASSES)])
+
+
+def write_storms(out, storms: list, f: int = 1, compress: bool = False, forcing_steps: int = 96,
+                 output_interval_s: float | None = None) -> dict:
+    """Run ``storms`` -- a list of (sim_id, SiteState, Scenario, meta dict) -- as one batch through
+    ``api.simulate_batch(sites, scenarios)``, as generate.py:288 calls it, and write each storm's record
+    and index entry (the loop body of generate.py:229-321) over the configured domain.  Resumable: a
+    storm whose record exists and which the index holds is skipped, and so is one recorded as failed.
+    A batch of one that the engine cannot simulate is recorded as failed; in a larger batch the
+    failing member cannot be told apart and the error propagates."""
     from . import api
+    from .scenarios import sea_level_rise, rain_factor
     from .solver.coupling import NumericalInstabilityError
+    ctx = api.context()
+    dom = ctx.domain
     out = Path(out)
     out.mkdir(parents=True, exist_ok=True)
     index_path = out / "index.json"
     index = json.loads(index_path.read_text(encoding="utf-8")) if index_path.exists() else {}
-    todo = [k for k, (i, _, _) in enumerate(storms)
-            if not (record_path(out, i).exists() and str(i) in index) and index.get(str(i), {}).get("status") != FAILED]
-    if not todo:
+    run = [st for st in storms
+           if not (record_path(out, st[0]).exists() and str(st[0]) in index)
+           and index.get(str(st[0]), {}).get("status") != FAILED]
+    if not run:
         return index
-    if len(todo) != len(storms):
-        from .forcing import select
-        forcing = dataclasses.replace(forcing, rain=select(forcing.rain, todo), inflow=select(forcing.inflow, todo),
-                                      tide=select(forcing.tide, todo))
-    run = [storms[k] for k in todo]
-    sites = [s for _, s, _ in run]
+    sites, scens = [st[1] for st in run], [st[2] for st in run]
     t0 = time.perf_counter()
     try:
-        results = api.simulate_batch(sites, forcing, terrain, network, output_interval_s, cfg, device)
+        results = api.simulate_batch(sites, scens, output_interval_s)
     except NumericalInstabilityError as e:
         if len(run) > 1:
             raise
-        i, _, meta = run[0]
+        i, _, _, meta = run[0]
         index[str(i)] = {**meta, "status": FAILED, "error": str(e), "wall_s": time.perf_counter() - t0}
         index_path.write_text(json.dumps(index, indent=1), encoding="utf-8")
         return index
     batch_wall = time.perf_counter() - t0
     wall = batch_wall / len(run)
-    for (i, site, meta), res in zip(run, results):
+    for (i, site, scen, meta), res in zip(run, results):
         path = record_path(out, i, compress)
-        save_record(simulation_record(res, f, terrain.dx, site=site, meta=meta, forcing_steps=forcing_steps,
-                                      wall_s=wall), path)
-        index[str(i)] = index_entry(path, res, terrain.sea, meta, f, wall, batch_wall, len(run))
+        lu = ctx.landuse(scen.ssp, scen.horizon_year)
+        meta = {**meta, "return_period_yr": scen.return_period_yr, "rcp": scen.rcp, "ssp": scen.ssp,
+                "horizon_year": scen.horizon_year, "tide_on": scen.tide_on,
+                "storm_tide_offset_h": scen.storm_tide_offset_h}
+        rec = simulation_record(res, f, dom.dx, site=site, meta=meta, landuse_frac=landuse_fractions(lu),
+                                horton_rate_mmh=horton_rate_at(lu, site.infil_multiplier.double().numpy(),
+                                                               res.times.numpy(), f),
+                                forcing_steps=forcing_steps, wall_s=wall,
+                                forcing_factors={"rain_factor": rain_factor(scen.rcp),
+                                                 "slr_m": sea_level_rise(scen.rcp, scen.horizon_year)})
+        save_record(rec, path)
+        index[str(i)] = index_entry(path, res, dom.sea, meta, f, wall, batch_wall, len(run))
         index_path.write_text(json.dumps(index, indent=1), encoding="utf-8")
     return index

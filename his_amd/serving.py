@@ -38,7 +38,7 @@ import torch
 
 from . import workloads as W
 from .config import SolverConfig
-from .engine import Forcing, build_engine_batch
+from .engine import Forcing, engine_from_fields
 from .forcing import PiecewiseLinear, sine_tide
 from .solver.coupling import (ERROR_CODES, MAX_PLAUSIBLE_DEPTH_2D_M, MAX_PLAUSIBLE_RISE_1D_M, CoupledEngine)
 
@@ -116,7 +116,7 @@ SERVING_GRIDS = (GridSpec(100, 81, 80.0, "chain"), GridSpec(200, 163, 40.0, "net
 def build_engine_for(spec: GridSpec, ter, net, members: list, forcing: Forcing, cfg: SolverConfig, device):
     cfg = cfg.replace(slot_width_frac=spec.slot_width_frac)
     flds = [member_fields(ter, m) for m in members]
-    return build_engine_batch(cfg, ter, net, forcing, [f[0] for f in flds], [f[1] for f in flds],
+    return engine_from_fields(cfg, ter, net, forcing, [f[0] for f in flds], [f[1] for f in flds],
                               [f[2] for f in flds], [f[3] for f in flds], [f[4] for f in flds],
                               [m.gamma for m in members], device=device, batched=True)
 

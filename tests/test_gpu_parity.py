@@ -465,37 +465,54 @@ def test_a_failing_request_fails_alone(gpu):
         assert np.array_equal(r.fields["h"], alone["h"]) and np.array_equal(r.fields["A"], alone["A"])
 
 
-def test_simulate_and_simulate_batch(gpu):
-    """FloodResult assembly: depth_max over the output-time snapshots, per-member mass balance,
-    a batch of one routed to simulate, and the zero disbenefit of a design against itself."""
+def _small_context(gpu):
+    """The API's context over the 100 x 81 @ 80 m plain with the 20-reach chain and its land use."""
     from his_amd import api
     from his_amd import workloads as W
-    from his_amd.engine import Forcing
-    wl = W.plain_cpu()
-    ter, net = wl.terrain, wl.network
-    wl.members = [wl.members[0], W.Member(rain_peak_mmh=60.0, tide_phase=0.7, inflow_peak=150.0)]
-    rain, tide, inflow = wl.forcing(batched=True)
-    fo2 = Forcing(rain, inflow, tide, wl.storm.t_end)
-    from his_amd.forcing import select
-    fo1 = Forcing(select(rain, [0]), select(inflow, [0]), select(tide, [0]), wl.storm.t_end)
-    site = api.baseline_site(ter, net)
-    r = api.simulate(site, fo1, ter, net, device=gpu)
-    nt = len(fo1.times(900.0))
-    assert r.depth_series.shape == (nt, ter.ny, ter.nx) and r.times.shape == (nt,)
-    assert torch.equal(r.depth_max, r.depth_series.amax(0))
-    assert r.mass_balance_error < 1e-3 and r.depth_max.max() > 0.1
-    assert r.channel_eta.shape[1] == int((r.extras["topology"].kind <= 1).sum())
-    (r1,) = api.simulate_batch([site], fo1, ter, net, device=gpu)
-    assert torch.equal(r1.depth_max, r.depth_max)
-    wetter = site.replace(storage_depth=site.storage_depth * 0.0)
-    rb = api.simulate_batch([site, wetter], fo2, ter, net, device=gpu)
-    assert len(rb) == 2 and all(x.mass_balance_error < 1e-3 for x in rb)
-    # member 0 of the batch shares dt with member 1, so it is close to, not equal to, the run alone
-    assert float((rb[0].depth_max - r.depth_max).abs().max()) < 0.05
-    d = api.disbenefit(r.depth_max, r.depth_max, ter, net)
-    assert d == {"max_depth_increase_m": 0.0, "area_worsened_ha": 0.0}
-    with pytest.raises(ValueError):
-        api.simulate(site.replace(manning=site.manning * 0.0), fo1, ter, net, device=gpu)
+    from his_amd.domain.site import domain_from
+    ter = W.make_terrain(100, 81, 80.0)
+    return api.configure(domain_from(ter, W.chain_network(ter, 20)), device=gpu)
+
+
+def test_simulate_and_simulate_batch(gpu):
+    """The reference's call surface: ``simulate(site, scenario, ...)`` and ``simulate_batch(sites,
+    scenarios)`` called as generate.py:288 calls it.  depth_max is the maximum over the output-time
+    snapshots, mass balance holds per member, a batch of one is routed to simulate, the disbenefit of
+    a design against itself is zero, and a bad site raises ValueError."""
+    from his_amd import api
+    from his_amd.scenarios import Scenario
+    ctx = _small_context(gpu)
+    dom = ctx.domain
+    try:
+        sc1 = Scenario(50, "RCP4.5", "SSP2", 2050)
+        sc2 = Scenario(100, "RCP8.5", "SSP5", 2080, tide_on=False)
+        site = api.baseline_site(sc1)
+        r = api.simulate(site, sc1)
+        nt = len(r.extras["forcing"].times(900.0))
+        assert r.depth_series.shape == (nt, dom.ny, dom.nx) and r.times.shape == (nt,)
+        assert torch.equal(r.depth_max, r.depth_series.amax(0))
+        assert r.mass_balance_error < 1e-3 and float(r.depth_max.max()) > 0.1 and r.infiltrated_volume_m3 > 0
+        assert r.channel_eta.shape[1] == int((r.extras["topology"].kind <= 1).sum())
+        assert "exposure" in r.extras and r.extras["exposure"].shape == dom.shape
+        (r1,) = api.simulate_batch([site], [sc1])
+        assert torch.equal(r1.depth_max, r.depth_max)
+        wetter = site.replace(storage_depth=site.storage_depth * 0.0)
+        rb = api.simulate_batch([site, wetter], [sc1, sc2])
+        assert len(rb) == 2 and all(x.mass_balance_error < 1e-3 for x in rb)
+        assert rb[1].extras["forcing"].ssp == "SSP5" and rb[0].extras["batch_size"] == 2
+        # member 0 of the batch shares dt with member 1, so it is close to, not equal to, the run alone
+        assert float((rb[0].depth_max - r.depth_max).abs().max()) < 0.05
+        # the wetter, heavier storm floods more
+        assert float(rb[1].depth_max.sum()) > float(rb[0].depth_max.sum())
+        assert api.disbenefit(r.depth_max, r.depth_max) == {"max_depth_increase_m": 0.0, "area_worsened_ha": 0.0}
+        rc = api.simulate(wetter, sc1, compare_to_baseline=True)
+        assert rc.max_depth_increase_m >= 0.0 and rc.area_worsened_ha >= 0.0
+        with pytest.raises(ValueError):
+            api.simulate(site.replace(manning=site.manning * 0.0), sc1)
+        with pytest.raises(ValueError):
+            api.simulate_batch([site, site], [sc1])
+    finally:
+        api.configure()
 
 
 @pytest.mark.parametrize("precision", ["fp32bulk", "fp64"])
