@@ -63,20 +63,25 @@ class Forcing:
 
 def normal_depth(one: SWE1D, Q, slope_min: float = 1e-4) -> torch.Tensor:
     """Manning normal depth for discharge Q in every cell, by bisection (engine.py:78-101)."""
-    tp = one.topo
-    z = one.z.detach().cpu().numpy()
+    sec = one.sec
+    y = normal_depth_arrays(one.topo, one.net.n_reaches, one.z.detach().cpu().numpy(), sec.b.cpu().numpy(),
+                            sec.m.cpu().numpy(), sec.yb.cpu().numpy(), sec.ts.cpu().numpy(), one.nman.cpu().numpy(),
+                            Q, one.bshape, slope_min)
+    return torch.as_tensor(y, dtype=torch.float64, device=one.device)
+
+
+def normal_depth_arrays(tp, n_reaches: int, z, b, m, yb, ts, nman, Q, bshape=(), slope_min: float = 1e-4):
+    """``normal_depth`` on host arrays: the bed slope along each reach's nodes (at least slope_min),
+    then 60 bisection steps of Manning's equation between 0 and the bankfull depth."""
     s = np.full(z.shape, slope_min)
-    for ri in range(one.net.n_reaches):
+    for ri in range(n_reaches):
         idx = np.nonzero(tp.reach_of == ri)[0]
         idx = idx[tp.kind[idx] == 0]
         if len(idx) > 1:
             dz = -np.gradient(z[..., idx], tp.chain[idx], axis=-1)
             s[..., idx] = np.maximum(dz, slope_min)
-    b, m = one.sec.b.cpu().numpy(), one.sec.m.cpu().numpy()
-    yb, ts = one.sec.yb.cpu().numpy(), one.sec.ts.cpu().numpy()
-    nman = one.nman.cpu().numpy()
     if np.ndim(Q):
-        Q = np.asarray(Q, dtype=float).reshape(one.bshape + (1,))
+        Q = np.asarray(Q, dtype=float).reshape(tuple(bshape) + (1,))
     lo, hi = np.zeros_like(s), np.broadcast_to(yb, s.shape).copy()
     for _ in range(60):
         mid = 0.5 * (lo + hi)
@@ -85,7 +90,7 @@ def normal_depth(one: SWE1D, Q, slope_min: float = 1e-4) -> torch.Tensor:
         R = A / (b + 2.0 * yt * np.sqrt(1.0 + m * m))
         q = A * R ** (2 / 3) * np.sqrt(s) / nman
         lo, hi = np.where(q < Q, mid, lo), np.where(q < Q, hi, mid)
-    return torch.as_tensor(0.5 * (lo + hi), dtype=torch.float64, device=one.device)
+    return 0.5 * (lo + hi)
 
 
 def _solver_cfg(cfg) -> SolverConfig:

@@ -126,18 +126,57 @@ def request_engine(spec: GridSpec, ter, net, req: dict, cfg: SolverConfig, devic
     return build_engine_for(spec, ter, net, [req["member"]], request_forcing(req), cfg, device)
 
 
+def outlets(net, tp) -> tuple:
+    """(cells, signs, main): the interior cells that drain through a stage end into the sea, the sign
+    that makes their discharge positive seaward, and the index among them of the river mouth (the
+    stage outlet reached downstream from the inflow reach)."""
+    stage = np.flatnonzero(tp.kind == 3)
+    cells, signs = tp.ghost_src[stage], tp.ghost_sign[stage].astype(np.float64)
+    r = net.reach_index(net.inflow_reach) if net.inflow_reach else 0
+    for _ in range(net.n_reaches):
+        down = net.reaches[r].down
+        if down[0] != "junction":
+            break
+        nxt = [k for k, q in enumerate(net.reaches) if q.up == down]
+        if not nxt:
+            break
+        r = nxt[0]
+    mouth = [k for k, c in enumerate(cells) if tp.reach_of[c] == r]
+    return cells, signs, (mouth[0] if mouth else 0)
+
+
+def sea_discharge(Q: np.ndarray, cells, signs, main: int):
+    """(total channel discharge into the sea, the river mouth's) from a member's 1-D discharge."""
+    q = Q[cells] * signs
+    return float(q.sum()), float(q[main]) if len(q) else 0.0
+
+
 def run_alone(spec: GridSpec, req: dict, cfg: SolverConfig, device):
     """A request solved by itself (its own engine), with the same outputs the server returns."""
     ter, net = spec.build()
     eng = request_engine(spec, ter, net, req, cfg, device)
     hmax = [None]
+    cells, signs, main = outlets(net, eng.one.topo)
+    hydro = []
 
     def snap(s):
         hmax[0] = s.h[0] if hmax[0] is None else np.maximum(hmax[0], s.h[0])
+        hydro.append((s.t,) + sea_discharge(eng.one.Q[0].detach().cpu().numpy(), cells, signs, main))
     dur = storm_length(req["storm_s"])
     err = eng.run(dur, out_times=output_times(dur), on_snapshot=snap)
     return dict(h=eng.two.h[0].detach().cpu().numpy(), A=eng.one.A[0].detach().cpu().numpy(),
-                depth_max=hmax[0], steps=eng.log.steps, mass_err=float(np.asarray(err).reshape(-1)[0]))
+                depth_max=hmax[0], steps=eng.log.steps, mass_err=float(np.asarray(err).reshape(-1)[0]),
+                hydrograph=hydro)
+
+
+def slot_bytes(spec: GridSpec, precision: str = "fp32bulk") -> int:
+    """HBM one busy slot of a grid size holds: 2-D state (5 fields), parameter fields (5), stage
+    temporaries (3), the running maximum depth, and the 1-D network's state, sections and workspace."""
+    ter, net = spec.build()
+    from .solver.swe1d import build_topology
+    item = 8 if precision == "fp64" else 4
+    nc = build_topology(net).n_cells
+    return ter.nx * ter.ny * item * (5 + 5 + 3 + 1) + nc * 8 * 12
 
 
 class Pool:
@@ -145,6 +184,7 @@ class Pool:
 
     def __init__(self, spec: GridSpec, slots: int, cfg: SolverConfig, device, seed: int):
         self.spec, self.B = spec, slots
+        self.cfg = cfg.replace(slot_width_frac=spec.slot_width_frac)
         self.ter, self.net = spec.build()
         members = W.draw_members(slots, seed)
         zeros = np.zeros((slots, K_KNOTS))
@@ -155,9 +195,70 @@ class Pool:
         two = self.eng.two
         self.cells = two.nx * two.ny
         self.hmax = torch.zeros_like(two.h)
-        # bytes a busy slot holds: 2-D state, parameter fields, stage temporaries, max depth, 1-D arrays
-        item = two.h.element_size()
-        self.slot_bytes = self.cells * item * (5 + 5 + 3 + 1) + self.eng.one.topo.n_cells * 8 * 12
+        self.slot_bytes = slot_bytes(spec, cfg.precision)
+        dev = two.device
+        self.sea = torch.as_tensor(self.ter.sea, device=dev)
+        self.land = ~self.sea
+        tp = self.eng.one.topo
+        self.out_cells, self.out_signs, self.out_main = outlets(self.net, tp)
+        self.interior = self.eng.one.interior.reshape(1, -1)
+
+    def load_request(self, b: int, req: dict, fo: Forcing) -> float:
+        """Load a request into slot b straight from its member draw, with the operations a standalone
+        build of it performs (``request_engine``), so the slot starts from the same bits: the 2-D
+        parameter fields and state, the channel sections of its conveyance, the channel at normal depth
+        (not below the tide).  Returns the storm's initial total volume (float64)."""
+        from .domain.channels import apply_gamma
+        from .engine import normal_depth_arrays
+        from .solver.swe1d import build_topology, section_arrays
+        m = req["member"]
+        eng = self.eng
+        two, one = eng.two, eng.one
+        f64 = dict(dtype=torch.float64, device=two.device)
+        zero = torch.zeros((), **f64)
+        on_land = lambda v: torch.where(self.land, torch.tensor(float(v), **f64), zero)
+        two.n[b].fill_(m.manning)
+        two.storage[b].copy_(on_land(m.retention_m))
+        two.horton.f0[b].copy_(on_land(m.f0_mmh * W.MMH))
+        two.horton.fc[b].copy_(on_land(m.fc_mmh * W.MMH))
+        two.horton.k[b].copy_(on_land(m.k_per_h / 3600.0))
+        eta0 = np.asarray(fo.tide(0.0), dtype=np.float64).reshape(-1)
+        eta0_2d = torch.as_tensor(eta0, dtype=two.dtype, device=two.device).reshape(1, 1)
+        two.h[b].copy_(torch.where(self.sea, torch.clamp(eta0_2d - two.z, min=0.0), torch.zeros_like(two.z)))
+        for name in ("hu", "hv", "s_filled", "infil_cum"):
+            getattr(two, name)[b].zero_()
+        # the member's network: conveyance-scaled sections, then the level of normal depth or the tide
+        tp = build_topology(apply_gamma(self.net, m.gamma))
+        z1, b1 = tp.z[None], tp.b[None]
+        sa = section_arrays(tp, z1, b1, self.cfg.slot_width_frac)
+        T = lambda a: torch.as_tensor(np.ascontiguousarray(a), **f64)
+        one.z[b].copy_(T(z1[0]))
+        for sec, fields in ((one.sec, (("b", b1), ("yb", sa["yb"]), ("ts", sa["ts"]))),
+                            (one.fsec, (("b", sa["fb"]), ("yb", sa["fyb"]), ("ts", sa["fts"])))):
+            for name, val in fields:
+                dst = getattr(sec, name)
+                if dst.dim() == 2:
+                    dst[b].copy_(T(val[0]))
+                elif not torch.equal(dst, T(val[0])):
+                    raise ValueError(f"section field {name} is shared in the pool but differs for the request")
+        yn = normal_depth_arrays(tp, self.net.n_reaches, z1, b1, tp.m, sa["yb"], sa["ts"], tp.n,
+                                 fo.inflow(0.0), (1,))
+        zb = T(z1)
+        level = torch.maximum(zb + T(yn), torch.as_tensor(eta0, **f64).reshape(1, 1).expand_as(zb))
+        y = torch.clamp(level - zb, min=0.0)
+        low, high = torch.minimum(y, T(sa["yb"])), torch.clamp(y - T(sa["yb"]), min=0.0)
+        A = low * (T(b1) + T(tp.m)[None] * low) + T(sa["ts"]) * high
+        one.A[b].copy_(A[0])
+        one.Q[b].zero_()
+        self.hmax[b].copy_(two.h[b])
+        vol = two._volumes()
+        v1 = (one.A[b:b + 1] * one.dx * self.interior).to(torch.float64).sum(-1)
+        return float((vol[0, b:b + 1] + vol[1, b:b + 1] + v1)[0])
+
+    def discharge_to_sea(self, members: list) -> dict:
+        """{slot: (total seaward channel discharge, river mouth's)} for the given slots, one copy."""
+        Q = self.eng.one.Q.detach().cpu().numpy()
+        return {b: sea_discharge(Q[b], self.out_cells, self.out_signs, self.out_main) for b in members}
 
     def load(self, b: int, req_eng: CoupledEngine):
         """Copy a one-member engine's state, fields and sections into slot b (in place)."""
@@ -195,6 +296,8 @@ class Result:
     mass_err: float = 0.0
     max_depth_m: float = 0.0
     wet_cells: int = 0
+    # (t, total channel discharge into the sea, the river mouth's) at every output time, t = 0 included
+    hydrograph: list = field(default_factory=list)
     fields: dict = field(default_factory=dict)
 
     @property
@@ -205,12 +308,23 @@ class Result:
 class Server:
     """Slot pools for the serving grid sizes in one device engine, fed by a request stream."""
 
-    def __init__(self, slots=(24, 16, 16), precision: str = "fp32bulk", hbm_budget_bytes: float = 32e9,
-                 device=None, grids=SERVING_GRIDS, chunk: int = 16, keep_fields: bool = False, tune: bool = True):
+    def __init__(self, slots=None, precision: str = "fp32bulk", hbm_budget_bytes: float | None = None,
+                 device=None, grids=SERVING_GRIDS, chunk: int = 16, keep_fields: bool = False, tune: bool = True,
+                 max_slots=(24, 16, 16)):
+        """``hbm_budget_bytes`` (default: a quarter of the device's memory) bounds the HBM that busy
+        slots may hold; a request is admitted only while it fits.  ``slots`` per grid size default to
+        what an equal share of the budget holds, at most ``max_slots``."""
         from .batching import GroupedEngine
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.cfg = SolverConfig().replace(precision=precision)
         self.grids = grids
+        if hbm_budget_bytes is None:
+            hbm_budget_bytes = 0.25 * torch.cuda.get_device_properties(self.device).total_memory
+        self.budget = float(hbm_budget_bytes)
+        if slots is None:
+            share = self.budget / len(grids)
+            slots = tuple(max(1, min(int(cap), int(share // slot_bytes(g, precision))))
+                          for g, cap in zip(grids, max_slots))
         self.pools = [Pool(g, s, self.cfg, self.device, seed=100 + i) for i, (g, s) in enumerate(zip(grids, slots))]
         if tune:
             from . import tuning
@@ -218,7 +332,6 @@ class Server:
         self.ge = GroupedEngine([p.eng for p in self.pools], dt_group_sizes=[1] * len(self.pools))
         self.dev = self.ge.dev
         self.m0 = self.ge.m0
-        self.budget = float(hbm_budget_bytes)
         self.chunk = int(chunk)
         self.keep_fields = keep_fields
         Bt = self.dev.Btot
@@ -240,10 +353,9 @@ class Server:
             return False
         b = pool.free.popleft()
         m = self.m0[pi] + b
-        reng = request_engine(pool.spec, pool.ter, pool.net, req, self.cfg, self.device)
-        pool.load(b, reng)
-        tab = self.dev.tables[pi]
         fo = request_forcing(req)
+        v0 = pool.load_request(b, req, fo)
+        tab = self.dev.tables[pi]
         tab["rain"]["vals"][b].copy_(torch.as_tensor(fo.rain.values[0], device=self.device))
         tab["inflow"]["vals"][b].copy_(torch.as_tensor(fo.inflow.values[0], device=self.device))
         for key in ("stage", "east"):
@@ -266,23 +378,28 @@ class Server:
         mask[m] = 1
         d.refresh(mask)
         res = Result(req["id"], pi, req["arrival_s"], admitted_s=now, t_end=fo.t_end)
-        self.busy[m] = (pi, b, res, float(np.asarray(reng.mb.v0).reshape(-1)[0]))
+        res.hydrograph.append((0.0, 0.0, 0.0))     # channels start at rest (Q = 0)
+        self.busy[m] = (pi, b, res, v0)
         self.resident += pool.slot_bytes
         return True
 
     # ------------------------------------------------------------ completion
     def _output(self, members: list, now: float, done: list):
         """Members at an output time: fold depth maxima, check plausibility, finish or move on."""
-        pl = {}
+        pl, qs = {}, {}
+        t_now = self.dev.t.cpu().numpy()
         for pi, pool in enumerate(self.pools):
-            if any(self.busy[m][0] == pi for m in members):
+            mine = [self.busy[m][1] for m in members if self.busy[m][0] == pi]
+            if mine:
                 pl[pi] = pool.eng.plausibility()
+                qs[pi] = pool.discharge_to_sea(mine)
         finished = []
         for m in members:
             pi, b, res, v0 = self.busy[m]
             pool = self.pools[pi]
             two = pool.eng.two
             torch.maximum(pool.hmax[b], two.h[b], out=pool.hmax[b])
+            res.hydrograph.append((float(t_now[m]),) + qs[pi][b])
             dmax, rise = pl[pi][0][b], pl[pi][1][b]
             if not (np.isfinite(dmax) and np.isfinite(rise) and dmax <= MAX_PLAUSIBLE_DEPTH_2D_M
                     and rise <= MAX_PLAUSIBLE_RISE_1D_M):

@@ -216,4 +216,33 @@ def test_served_mix_with_full_size_requests_equals_each_alone(gpu):
         assert np.array_equal(r.fields["h"], alone["h"]), f"request {req['id']} h"
         assert np.array_equal(r.fields["A"], alone["A"]), f"request {req['id']} A"
         assert np.array_equal(r.fields["depth_max"], alone["depth_max"]), f"request {req['id']} depth max"
+        assert r.hydrograph == alone["hydrograph"], f"request {req['id']} outlet hydrograph"
         assert abs(r.mass_err - alone["mass_err"]) <= 1e-12 + 1e-9 * alone["mass_err"]
+    big = [res[q["id"]] for q in reqs if q["size"] == 2]
+    assert all(max(x[1] for x in r.hydrograph) > 1.0 for r in big), "water reaches the sea through the channels"
+
+
+def test_hbm_budget_holds_a_request_in_the_queue(gpu):
+    """A budget with room for one busy 200 x 163 slot: the second request waits, although a slot is
+    free, until the first has finished; each still equals its standalone run, hydrograph included."""
+    from his_amd import serving as S
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    grids = (S.GridSpec(200, 163, 40.0, "net60"),)
+    one_slot = S.slot_bytes(grids[0], "fp64")
+    ms = W.draw_members(2, 4321)
+    reqs = [dict(id=i, arrival_s=0.0, size=0, storm_s=600.0, member=ms[i]) for i in range(2)]
+    srv = S.Server(slots=(2,), precision="fp64", hbm_budget_bytes=1.5 * one_slot, device=gpu, grids=grids,
+                   chunk=16, keep_fields=True, tune=False)
+    res = {r.id: r for r in srv.serve(reqs, realtime=False, log_every_s=0)}
+    srv.close()
+    assert res[1].admitted_s >= res[0].done_s > res[0].admitted_s
+    cfg = SolverConfig().replace(precision="fp64")
+    for req in reqs:
+        r, alone = res[req["id"]], S.run_alone(grids[0], req, cfg, gpu)
+        assert r.status == "ok" and r.steps == alone["steps"]
+        assert np.array_equal(r.fields["h"], alone["h"]) and np.array_equal(r.fields["A"], alone["A"])
+        assert r.hydrograph == alone["hydrograph"]
+    with pytest.raises(RuntimeError, match="budget"):
+        S.Server(slots=(1,), precision="fp64", hbm_budget_bytes=0.5 * one_slot, device=gpu, grids=grids,
+                 tune=False).serve(reqs[:1], realtime=False, log_every_s=0)
