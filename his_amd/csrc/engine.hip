@@ -53,7 +53,10 @@ int launch_finalize(const FinalizeArgs& f, hipStream_t s) {
 
 // ------------------------------------------------------------------ building launches
 
-static int g_shape[2] = {0, 0};   // tile-shape index per dtype (HIS_F64, HIS_F32)
+// Tile-shape index per dtype (HIS_F64, HIS_F32).  The defaults are the shapes the autotuner picks
+// on the 400 x 325 grids: 32 x 8 LDS tiles in float64, the register march in 8-row chunks in float32.
+constexpr int kDefaultShape[2] = {4, 2};
+static int g_shape[2] = {kDefaultShape[0], kDefaultShape[1]};
 
 static int shape_of(int dtype, int* tx, int* ty) {
     const TileShape& t = dtype == HIS_F64 ? kShapesF64[g_shape[0]] : kShapesF32[g_shape[1]];
@@ -931,11 +934,12 @@ extern "C" int his_engine_destroy(his_engine* e) {
 }
 
 extern "C" int his_set_tile_shape(int32_t dtype, int32_t shape) {
-    if ((dtype != HIS_F64 && dtype != HIS_F32) || shape < 0 || shape >= kNumShapes) {
+    if ((dtype != HIS_F64 && dtype != HIS_F32) || shape < -1 || shape >= kNumShapes) {
         set_error("his_set_tile_shape: bad dtype or shape");
         return HIS_E_ARG;
     }
-    g_shape[dtype == HIS_F64 ? 0 : 1] = shape;
+    const int d = dtype == HIS_F64 ? 0 : 1;
+    g_shape[d] = shape < 0 ? kDefaultShape[d] : shape;
     return 0;
 }
 

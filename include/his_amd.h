@@ -269,7 +269,8 @@ int his_engine_destroy(his_engine *e);
 
 /* ------------------------------------------------------------------ misc */
 
-/* Tile shape of the 2-D kernels, per dtype: index into the compiled shapes (0 = default).
+/* Tile shape of the 2-D kernels, per dtype: index into the compiled shapes; -1 restores the
+ * default (float64: shape 4, 32 x 8 LDS tiles; float32: shape 2, register march in 8-row chunks).
  * The autotuner (his_amd/tuning.py) times each and stores the fastest per grid.
  * his_tile_shape reports a shape: tile or strip width tx, rows ty, and the kernel variant
  * (0 LDS-tiled, 1 LDS-ring march, 2 register march); variant may be NULL. */

@@ -121,5 +121,5 @@ def test_every_shape_on_a_grid_with_interior_strips(gpu, reference, shape):
                                             for k in range(3))))
             assert_rel(rates, rate0, RTOL if dtype == torch.float64 else 1e-5, 0, f"{dtype} shape {shape} rate")
     finally:
-        _lib.set_tile_shape(torch.float64, 0)
-        _lib.set_tile_shape(torch.float32, 0)
+        _lib.set_tile_shape(torch.float64, -1)
+        _lib.set_tile_shape(torch.float32, -1)

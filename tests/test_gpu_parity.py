@@ -358,8 +358,8 @@ def test_every_kernel_shape_matches_reference(golden, gpu, shape):
             rates = _f64(sim.member_rates(*(torch.as_tensor(st[0][k], dtype=dtype, device=gpu) for k in range(3))))
             assert_rel(rates, d["rate_member_0"], RTOL if dtype == torch.float64 else 1e-5, 0, f"shape {shape} rate")
     finally:
-        _lib.set_tile_shape(torch.float64, 0)
-        _lib.set_tile_shape(torch.float32, 0)
+        _lib.set_tile_shape(torch.float64, -1)
+        _lib.set_tile_shape(torch.float32, -1)
 
 
 def _small_groups(device, precision="fp64"):
@@ -544,7 +544,7 @@ def test_engine_follows_a_tile_shape_change(gpu):
     state and the ledger match an engine that used the new shape throughout."""
     from his_amd import _lib
     try:
-        _lib.set_tile_shape(torch.float64, 0)
+        _lib.set_tile_shape(torch.float64, -1)
         eng_a, _ = coupled_case(gpu)
         eng_a.run(12.0, out_times=[])
         _lib.set_tile_shape(torch.float64, 6)
@@ -553,7 +553,7 @@ def test_engine_follows_a_tile_shape_change(gpu):
         eng_b.run(12.0, out_times=[])
         eng_b.run(30.0, t0=12.0, out_times=[])
     finally:
-        _lib.set_tile_shape(torch.float64, 0)
+        _lib.set_tile_shape(torch.float64, -1)
     assert eng_a.log.steps == eng_b.log.steps
     for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
         assert_rel(_f64(getattr(eng_a.two, name)), _f64(getattr(eng_b.two, name)), 1e-12, 1e-14, name)
