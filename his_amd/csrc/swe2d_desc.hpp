@@ -23,6 +23,10 @@ struct Group2D {
     int B, ny, nx, order, flux;
     int bc[4];
     int tiles_x, tiles_y, tile0, member0;
+    // Dispatch order of the register march: strips in_s0..in_s1 and chunks in_r0..in_r1 are interior
+    // (no ghost cell, no boundary row or column); the other `ring` tiles of a member touch an edge.
+    // e0 / i0: the group's first index among all edge tiles / all interior tiles of the launch.
+    int in_s0, in_s1, in_r0, in_r1, ring, e0, i0;
     R dx, dy, g, h_dry, kp_eps;
     double dxd, dyd, gd, t0, area;
     const R *z, *n, *S, *f0, *fc, *k, *rw;
@@ -35,6 +39,7 @@ template <typename R>
 struct Launch2D {
     Group2D<R> g[HIS_MAX_GROUPS];
     int ngroups, ntiles, Btot;
+    int nedge;                    // tiles that touch a grid edge, over all groups
     int tx, ty;                   // tile shape used to build tiles_x / tiles_y
     const double *dt, *t, *pm;    // (Btot), (Btot), (9, Btot)
     const int32_t *active;        // (Btot) or NULL = every member steps
@@ -110,9 +115,65 @@ struct FinalizeArgs {
 };
 int launch_finalize(const FinalizeArgs& f, hipStream_t s);
 
+// blockIdx -> logical tile so that each XCD (blocks are dealt round-robin over 8) walks a
+// contiguous run of tiles and neighbouring tiles share that XCD's L2 for their halos
+__host__ __device__ inline int xcd_tile(int p, int n) {
+    int q = n >> 3, r = n & 7, x = p & 7;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (p >> 3);
+}
+
+// blockIdx -> tile for the register march.  A tile that touches a grid edge takes about 1.5 times
+// as long as an interior one (ghost layers, boundary sums), so blocks 0 .. nedge-1 take the edge
+// tiles of every group and the rest the interior tiles, each class spread over the XCDs as
+// xcd_tile does.  A slot that frees up gets the next block, so short interior tiles fill the end of
+// the launch.  Only the order changes: tile numbering, partials and their reduction are the same.
+template <typename R>
+__host__ __device__ inline int edge_first_tile(const Launch2D<R>& L, int p, int& gi) {
+    const bool edge = p < L.nedge;
+    const int k = edge ? xcd_tile(p, L.nedge) : xcd_tile(p - L.nedge, L.ntiles - L.nedge);
+    gi = 0;
+    if (edge) {
+        while (gi + 1 < L.ngroups && k >= L.g[gi + 1].e0) ++gi;
+    } else {
+        while (gi + 1 < L.ngroups && k >= L.g[gi + 1].i0) ++gi;
+    }
+    const Group2D<R>& G = L.g[gi];
+    const int X = G.tiles_x, per = X * G.tiles_y;
+    const int ns = G.in_s1 - G.in_s0 + 1;
+    int b, r, s;
+    if (edge) {
+        const int q = k - G.e0;
+        b = q / G.ring;
+        int c = q - b * G.ring;
+        const int below = G.in_r0 * X, above = (G.tiles_y - 1 - G.in_r1) * X;
+        if (G.ring == per || c < below) {
+            r = c / X; s = c % X;
+        } else if ((c -= below) < above) {
+            r = G.in_r1 + 1 + c / X; s = c % X;
+        } else {
+            c -= above;
+            const int w = X - ns;            // edge strips beside the interior rows (in_s0 >= 1)
+            r = G.in_r0 + c / w;
+            const int cc = c % w;
+            s = cc < G.in_s0 ? cc : G.in_s1 + 1 + (cc - G.in_s0);
+        }
+    } else {
+        const int q = k - G.i0;
+        const int inner = per - G.ring;
+        b = q / inner;
+        const int c = q - b * inner;
+        r = G.in_r0 + c / ns;
+        s = G.in_s0 + c % ns;
+    }
+    return G.tile0 + (b * G.tiles_y + r) * X + s;
+}
+
+// Tiles of each group for a tile shape, and the two dispatch classes of the register march: a tile
+// is interior when its columns x0-3 .. x0+tx+2 and rows y0-3 .. y0+ty+2 lie inside the grid, which
+// holds for strips 1 .. (nx-tx-3)/tx and chunks 1 .. (ny-ty-3)/ty (a rectangle, possibly empty).
 template <typename R>
 inline void tiles_for(Launch2D<R>& L, int tx, int ty) {
-    int t = 0;
+    int t = 0, ne = 0, ni = 0;
     L.tx = tx;
     L.ty = ty;
     for (int gi = 0; gi < L.ngroups; ++gi) {
@@ -120,9 +181,26 @@ inline void tiles_for(Launch2D<R>& L, int tx, int ty) {
         G.tiles_x = (G.nx + tx - 1) / tx;
         G.tiles_y = (G.ny + ty - 1) / ty;
         G.tile0 = t;
-        t += G.B * G.tiles_x * G.tiles_y;
+        const int per = G.tiles_x * G.tiles_y;
+        t += G.B * per;
+        G.in_s0 = 1;
+        G.in_s1 = G.nx - tx - 3 >= 0 ? (G.nx - tx - 3) / tx : 0;
+        G.in_r0 = 1;
+        G.in_r1 = G.ny - ty - 3 >= 0 ? (G.ny - ty - 3) / ty : 0;
+        const int ns = G.in_s1 - G.in_s0 + 1, nr = G.in_r1 - G.in_r0 + 1;
+        if (ns <= 0 || nr <= 0) {           // no interior tile: every tile is an edge tile
+            G.in_s1 = 0; G.in_r1 = 0;
+            G.ring = per;
+        } else {
+            G.ring = per - ns * nr;
+        }
+        G.e0 = ne;
+        G.i0 = ni;
+        ne += G.B * G.ring;
+        ni += G.B * (per - G.ring);
     }
     L.ntiles = t;
+    L.nedge = ne;
 }
 
 }  // namespace his
