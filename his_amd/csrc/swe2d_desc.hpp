@@ -61,9 +61,10 @@ struct Launch2D {
 //   march = 1: one wave marches up a strip of TX = 58 columns for TY rows with LDS row rings (k_march)
 //   march = 2: the same walk with each lane's column held in registers, x-neighbours by shuffle (k_wave)
 struct TileShape { int tx, ty, nt, march; };
-// Chunk heights between 8 and 16 trade the march's warm-up rows (H + 6 row steps for H rows)
-// against waves per SIMD; on 400 x 325 grids H = 13 divides the rows exactly.
-constexpr TileShape kShapesF32[] = {{58, 16, 64, 2}, {58, 32, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
+// Chunk heights between 8 and 19 trade the march's warm-up rows (H + 6 row steps for H rows)
+// against rounds of waves: on 400 x 325 grids H = 13 divides the rows exactly, and at 32 members
+// H = 19 makes 4,032 waves, one round of the 4,096 wave slots (1,024 SIMDs x 4).
+constexpr TileShape kShapesF32[] = {{58, 16, 64, 2}, {58, 19, 64, 2}, {58, 8, 64, 2}, {58, 16, 64, 1},
                                     {64, 8, 256, 0}, {64, 16, 256, 0}, {58, 10, 64, 2}, {58, 12, 64, 2},
                                     {58, 13, 64, 2}};
 // float64 runs mostly at B = 1, where small tiles are what fills the chip

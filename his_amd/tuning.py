@@ -44,25 +44,29 @@ def key(dtype, nx, ny, B) -> str:
     return f"{'f64' if dtype == torch.float64 else 'f32'}:{nx}:{ny}:{B}"
 
 
-def time_shapes(two, reps: int = 20) -> dict:
-    """Mean microseconds of the 2-D step for every shape, on a copy of ``two``'s state."""
+def time_shapes(two, reps: int = 20, passes: int = 2) -> dict:
+    """Microseconds of the 2-D step for every shape, on a copy of ``two``'s state: the mean over
+    ``reps`` steps, the smallest such mean over ``passes`` sweeps of all shapes (clock and
+    neighbour drift between sweeps otherwise decide between shapes a few percent apart)."""
     two._sync_state()
     st = (two.h.clone(), two.hu.clone(), two.hv.clone(), two.s_filled.clone(), two.infil_cum.clone())
     pm = two._pack([0.0] * 9)
     dt_t = two._pack([0.05, 0.0])
     out = {}
-    for shape in range(L.num_tile_shapes()):
-        L.set_tile_shape(two.dtype, shape)
-        for _ in range(3):
-            two._run(*st, dt_t, pm)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            two._run(*st, dt_t, pm)
-        e1.record()
-        torch.cuda.synchronize()
-        out[shape] = 1e3 * e0.elapsed_time(e1) / reps
+    for _ in range(passes):
+        for shape in range(L.num_tile_shapes()):
+            L.set_tile_shape(two.dtype, shape)
+            for _ in range(3):
+                two._run(*st, dt_t, pm)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                two._run(*st, dt_t, pm)
+            e1.record()
+            torch.cuda.synchronize()
+            us = 1e3 * e0.elapsed_time(e1) / reps
+            out[shape] = min(us, out.get(shape, us))
     return out
 
 
@@ -79,6 +83,52 @@ def tune(two, reps: int = 20, force: bool = False) -> int:
         _save(tab)
     L.set_tile_shape(two.dtype, best)
     return best
+
+
+def tune_coupled(eng, steps: int = 60, warm: int = 4) -> tuple[int, dict]:
+    """Select the shape whose coupled step is fastest: the engine's own graph-replayed step, with
+    the 1-D network on its side stream beside the 2-D stages, which holds CUs the 2-D waves would
+    otherwise take (a shape whose waves fill the chip in one round alone can need a second round
+    beside it).  The timing steps are real steps of the run: the storm advances ``warm + steps``
+    steps per shape, and the engine's ledger and step count take them in.  Sets the shape active
+    and returns it with the milliseconds per coupled step of every shape."""
+    two = eng.two
+    dev = eng._device()
+    dev.refresh()
+    t_end0, t_lim0 = dev.t_end.clone(), dev.t_limit.clone()
+    dev.t_end.fill_(1e12)
+    dev.t_limit.fill_(1e12)
+    s0 = int(dev.steps[0].item())
+    times = {}
+    try:
+        for shape in range(L.num_tile_shapes()):
+            L.set_tile_shape(two.dtype, shape)
+            dev.step(warm)                   # the engine takes the new tiles and recaptures its graphs
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            dev.step(steps)
+            e1.record()
+            torch.cuda.synchronize()
+            times[shape] = e0.elapsed_time(e1) / steps
+    finally:
+        dev.t_end.copy_(t_end0)
+        dev.t_limit.copy_(t_lim0)
+    eng.log.steps += int(dev.steps[0].item()) - s0
+    eng._fold_device_ledger()
+    best = int(min(times, key=times.get))
+    L.set_tile_shape(two.dtype, best)
+    tab = _load()
+    tab[key(two.dtype, two.nx, two.ny, two.B)] = {"shape": best, "us": 1e3 * times[best], "coupled": True,
+                                                  "all_us": {str(s): 1e3 * t for s, t in times.items()}}
+    _save(tab)
+    return best, times
+
+
+def timings(two) -> dict:
+    """The remembered per-shape microseconds of this solver's grid ({} if it was never timed)."""
+    rec = _load().get(key(two.dtype, two.nx, two.ny, two.B), {})
+    return {int(k): v for k, v in rec.get("all_us", {}).items()}
 
 
 def describe(dtype, shape: int) -> str:
