@@ -59,7 +59,8 @@ def engine_throughput(wl, steps: int, warmup: int, t_start: float):
     shape, times = tuning.tune_coupled(eng)
     de = eng._device()
     t_now = float(de.t[0].item())
-    log(f"[bench]   kernel shape {shape} ({tuning.describe(eng.two.dtype, shape)}), coupled step per shape: "
+    log(f"[bench]   kernel shape {shape} ({tuning.describe(eng.two.dtype, shape)}), branch captured first: "
+        f"{('2-D', '1-D')[eng.branch_order] if eng.branch_order >= 0 else 'estimate'}, coupled step per shape: "
         + ", ".join(f"{s}:{1e3 * t:.0f}us" for s, t in times.items()) + f"; tuned over t = {t_tune:.0f}..{t_now:.0f} s")
     if t_now < t_start:
         eng.run(t_start, out_times=[], t0=t_now, progress_s=0)

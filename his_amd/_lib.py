@@ -97,6 +97,7 @@ SYMBOLS = {
     "his_engine_refresh_members": (C.c_int, [VP, VP, VP]),
     "his_engine_fold_max": (C.c_int, [VP, C.c_int32, VP, VP]),
     "his_engine_track_step_max": (C.c_int, [VP, C.c_int32, VP]),
+    "his_engine_set_branch_order": (C.c_int, [VP, C.c_int32]),
     "his_engine_destroy": (C.c_int, [VP]),
     "his_set_tile_shape": (C.c_int, [C.c_int32, C.c_int32]),
     "his_tile_shape": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),

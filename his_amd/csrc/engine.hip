@@ -420,6 +420,7 @@ struct his_engine {
     FinalizeArgs* fin_dev;    // device copy of F for that deferred reduction
     int fuse_head;            // dt and forcing evaluated inside the first exchange phase
     int one_first;            // capture the 1-D network before the 2-D stages (the 1-D branch is longer)
+    int one_first_est;        // the same from the cost estimate at creation (his_engine_set_branch_order -1)
     int ex_one;               // every 1-D cell has at most one link: the exchange is one launch
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
@@ -743,6 +744,7 @@ extern "C" int his_engine_create(const his_engine_desc* d, his_engine** out, voi
         }
         const double est_2d_us = cells / (dtype == HIS_F64 ? 4.0e3 : 2.3e4);
         e->one_first = est_2d_us < est_1d_us;
+        e->one_first_est = e->one_first;
     }
     D.Btot = Btot;
     for (int gi = 0; gi < d->ngroups; ++gi) {
@@ -917,6 +919,16 @@ extern "C" int his_engine_track_step_max(his_engine* e, int32_t group, void* hma
     if (!e || group < 0 || group >= e->d.ngroups) { set_error("his_engine_track_step_max: bad argument"); return HIS_E_ARG; }
     if (e->step_max[group] != hmax) {
         e->step_max[group] = hmax;
+        engine_drop_graphs(e);
+    }
+    return 0;
+}
+
+extern "C" int his_engine_set_branch_order(his_engine* e, int32_t order) {
+    if (!e || order < -1 || order > 1) { set_error("his_engine_set_branch_order: bad argument"); return HIS_E_ARG; }
+    const int v = order < 0 ? e->one_first_est : order;
+    if (v != e->one_first) {
+        e->one_first = v;
         engine_drop_graphs(e);
     }
     return 0;

@@ -539,7 +539,10 @@ def main():
         Path(a.out).write_text(json.dumps(dict(summary=summ, requests=[
             dict(id=r.id, size=r.size, arrival_s=r.arrival_s, admitted_s=r.admitted_s, done_s=r.done_s,
                  latency_s=r.latency_s, t_end_s=r.t_end, steps=r.steps, status=r.status, mass_err=r.mass_err,
-                 max_depth_m=r.max_depth_m, wet_cells=r.wet_cells) for r in res]), indent=0))
+                 max_depth_m=r.max_depth_m, wet_cells=r.wet_cells, hydrograph_samples=len(r.hydrograph),
+                 peak_sea_discharge_m3s=max((q[1] for q in r.hydrograph), default=0.0),
+                 peak_mouth_discharge_m3s=max((q[2] for q in r.hydrograph), default=0.0))
+            for r in res]), indent=0))
 
 
 if __name__ == "__main__":

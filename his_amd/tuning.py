@@ -89,9 +89,11 @@ def tune_coupled(eng, steps: int = 60, warm: int = 4) -> tuple[int, dict]:
     """Select the shape whose coupled step is fastest: the engine's own graph-replayed step, with
     the 1-D network on its side stream beside the 2-D stages, which holds CUs the 2-D waves would
     otherwise take (a shape whose waves fill the chip in one round alone can need a second round
-    beside it).  The timing steps are real steps of the run: the storm advances ``warm + steps``
-    steps per shape, and the engine's ledger and step count take them in.  Sets the shape active
-    and returns it with the milliseconds per coupled step of every shape."""
+    beside it).  With a network, both capture orders of the two branches are timed for each shape
+    (the branch captured second pays a cross-queue start).  The timing steps are real steps of
+    the run: the storm advances ``warm + steps`` steps per candidate, and the engine's ledger and
+    step count take them in.  Sets the best shape and order and returns the shape with the
+    milliseconds per coupled step of every shape (its better order)."""
     two = eng.two
     dev = eng._device()
     dev.refresh()
@@ -99,18 +101,23 @@ def tune_coupled(eng, steps: int = 60, warm: int = 4) -> tuple[int, dict]:
     dev.t_end.fill_(1e12)
     dev.t_limit.fill_(1e12)
     s0 = int(dev.steps[0].item())
-    times = {}
+    orders = (0, 1) if getattr(eng, "one", None) is not None else (-1,)
+    times, order_of = {}, {}
     try:
         for shape in range(L.num_tile_shapes()):
             L.set_tile_shape(two.dtype, shape)
-            dev.step(warm)                   # the engine takes the new tiles and recaptures its graphs
-            torch.cuda.synchronize()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            dev.step(steps)
-            e1.record()
-            torch.cuda.synchronize()
-            times[shape] = e0.elapsed_time(e1) / steps
+            for order in orders:
+                dev.set_branch_order(order)
+                dev.step(warm)               # the engine takes the new tiles and recaptures its graphs
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                dev.step(steps)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                if shape not in times or ms < times[shape]:
+                    times[shape], order_of[shape] = ms, order
     finally:
         dev.t_end.copy_(t_end0)
         dev.t_limit.copy_(t_lim0)
@@ -118,10 +125,13 @@ def tune_coupled(eng, steps: int = 60, warm: int = 4) -> tuple[int, dict]:
     eng._fold_device_ledger()
     best = int(min(times, key=times.get))
     L.set_tile_shape(two.dtype, best)
+    dev.set_branch_order(order_of[best])
     tab = _load()
     tab[key(two.dtype, two.nx, two.ny, two.B)] = {"shape": best, "us": 1e3 * times[best], "coupled": True,
+                                                  "branch_order": order_of[best],
                                                   "all_us": {str(s): 1e3 * t for s, t in times.items()}}
     _save(tab)
+    eng.branch_order = order_of[best]
     return best, times
 
 

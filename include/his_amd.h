@@ -265,6 +265,13 @@ int his_engine_fold_max(his_engine *e, int32_t group, void *hmax, void *stream);
  * Cost: one pass over the group's depth per step. */
 int his_engine_track_step_max(his_engine *e, int32_t group, void *hmax);
 
+/* Which branch of a coupled step is captured first: 0 the 2-D stages, 1 the 1-D network,
+ * -1 the engine's own estimate of the longer one (the default).  In a captured graph the first
+ * branch keeps the exchange's hardware queue and the other pays a cross-queue start, so the longer
+ * branch should go first; the estimate can miss, and a tuner can time both.  The captured step
+ * graphs are captured again at the next step.  No counterpart in hydrointel (no graphs there). */
+int his_engine_set_branch_order(his_engine *e, int32_t order);
+
 int his_engine_destroy(his_engine *e);
 
 /* ------------------------------------------------------------------ misc */
