@@ -36,7 +36,7 @@ ALG_BYTES_F32 = 64
 ALG_BYTES_F64 = 128
 # VALU issue peak: one wave64 instruction per 4 cycles on each of 1,024 SIMDs at 2.4 GHz
 VALU_PEAK_WAVE_INSTR_PER_S = 1024 * 2.4e9 / 4
-TUNE_LEAD_S = 900.0         # the shape is tuned on coupled steps from t_start - 15 min
+TUNE_LEAD_S = 1800.0        # the shape is tuned on coupled steps from t_start - 30 min
 T_START_S = 3 * 3600.0      # headline window starts at the rain peak, with the plain wet
 
 
@@ -51,8 +51,8 @@ def engine_throughput(wl, steps: int, warmup: int, t_start: float):
     dev = torch.device("cuda")
     eng, fo = build_from_workload(wl, device=dev)
     from his_amd import tuning
-    # the kernel shape is chosen on coupled steps of the wet storm a quarter hour before the timed
-    # window (the timing steps are steps of the storm), then the storm runs on to the window
+    # the kernel shape and capture order are chosen on coupled steps of the wet storm half an hour
+    # before the timed window (the timing steps are steps of the storm), then the storm runs on
     t_tune = max(0.0, t_start - TUNE_LEAD_S)
     if t_tune > 0:
         eng.run(t_tune, out_times=[], progress_s=0)
