@@ -346,10 +346,7 @@ def main():
         cpu = cpu_baseline(args.cpu_budget)
         log(f"[bench]   {cpu['value'] / 1e6:.2f} M cell-updates/s on {cpu['cores']} threads")
     valu_instr = _profile_figure(VALU_FILE, "valu_wave_instructions_per_step")
-    valu_trans = _profile_figure(VALU_FILE, "valu_transcendental_per_step") or 0
     valu_rate = valu_instr / (us * 1e-6) if valu_instr else None
-    # the same with each quarter-rate transcendental counted as four issue slots
-    valu_slots_rate = (valu_instr + 3 * valu_trans) / (us * 1e-6) if valu_instr else None
     out = {
         "metric": METRIC, "value": r32["rate"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -376,8 +373,6 @@ def main():
                      "valu": None if valu_rate is None else {
                          "achieved_wave_instr_per_s": valu_rate, "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S,
                          "frac": valu_rate / VALU_PEAK_WAVE_INSTR_PER_S, "wave_instr_per_step": valu_instr,
-                         "frac_transcendental_x4": valu_slots_rate / VALU_PEAK_WAVE_INSTR_PER_S,
-                         "transcendental_per_step": valu_trans,
                          "source": VALU_FILE.name}},
         "cpu_baseline": cpu,
     }
