@@ -581,3 +581,46 @@ def test_branch_order_does_not_change_results(gpu):
     sa, sb = a.mb.sources(), b.mb.sources()
     for k in sa:
         assert np.array_equal(np.asarray(sa[k]), np.asarray(sb[k])), k
+
+
+def test_coupled_tuner_steps_are_steps_of_the_run(gpu, tmp_path, monkeypatch):
+    """tune_coupled times every shape and capture order on the engine's own coupled steps: those
+    steps count in the run's step total and ledger (mass balance holds after them), the chosen
+    shape and order stay set, and the run continues to the same state as an engine that took the
+    same steps with the chosen settings throughout (float64: shape changes move only the order of
+    the float64 partial sums)."""
+    from his_amd import _lib, tuning
+    monkeypatch.setenv("HIS_AMD_TUNING", str(tmp_path / "tuning.json"))
+    try:
+        eng_a, _ = coupled_case(gpu)
+        eng_a.run(6.0, out_times=[])
+        shape, times = tuning.tune_coupled(eng_a, steps=3, warm=1)
+        assert sorted(times) == list(range(_lib.num_tile_shapes()))
+        assert shape == min(times, key=times.get)
+        de = eng_a._device()
+        t_now = float(de.t[0].item())
+        steps_a = eng_a.log.steps
+        assert steps_a == int(de.steps[0].item())
+        err = np.asarray(eng_a.mb.check(eng_a.total_volume(), t_now))
+        assert err.max() < 1e-12
+        eng_a.run(t_now + 20.0, t0=t_now, out_times=[])
+        # the same number of steps with the chosen shape and order from the start
+        _lib.set_tile_shape(torch.float64, shape)
+        eng_b, _ = coupled_case(gpu)
+        eng_b._device().set_branch_order(eng_a.branch_order)
+        eng_b.run(6.0, out_times=[])
+        db = eng_b._device()
+        db.t_end.fill_(1e12)
+        db.t_limit.fill_(1e12)
+        db.step(steps_a - eng_b.log.steps)
+        eng_b.log.steps = steps_a
+        eng_b._fold_device_ledger()
+        assert abs(float(db.t[0].item()) - t_now) < 1e-9
+        eng_b.run(t_now + 20.0, t0=t_now, out_times=[])
+    finally:
+        _lib.set_tile_shape(torch.float64, -1)
+    assert eng_a.log.steps == eng_b.log.steps
+    for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
+        assert_rel(_f64(getattr(eng_a.two, name)), _f64(getattr(eng_b.two, name)), 1e-12, 1e-14, name)
+    err = np.asarray(eng_a.mb.check(eng_a.total_volume(), t_now + 20.0))
+    assert err.max() < 1e-12
