@@ -196,7 +196,8 @@ def _horton(lu, kappa):
 
 def _snapshots(eng, t_end: float, times, label_: str):
     snaps = []
-    err = eng.run(t_end, out_times=times, on_snapshot=snaps.append, label=label_)
+    # the snapshots are only collected here, so their arrays may reach the host while the run steps on
+    err = eng.run(t_end, out_times=times, on_snapshot=snaps.append, label=label_, async_outputs=True)
     return snaps, err
 
 

@@ -624,3 +624,22 @@ def test_coupled_tuner_steps_are_steps_of_the_run(gpu, tmp_path, monkeypatch):
         assert_rel(_f64(getattr(eng_a.two, name)), _f64(getattr(eng_b.two, name)), 1e-12, 1e-14, name)
     err = np.asarray(eng_a.mb.check(eng_a.total_volume(), t_now + 20.0))
     assert err.max() < 1e-12
+
+
+def test_async_outputs_equal_sync_outputs(gpu):
+    """run(async_outputs=True) delivers every output, in order, with the same arrays and ledger
+    as synchronous snapshots of the same run."""
+    outs = [0.0, 5.0, 10.0, 15.0, 20.0, 25.0, 30.0]
+    got = {}
+    for mode in (False, True):
+        eng, _ = coupled_case(gpu)
+        snaps = []
+        eng.run(30.0, out_times=outs, on_snapshot=snaps.append, async_outputs=mode)
+        got[mode] = snaps
+    a, b = got[False], got[True]
+    assert [s.t for s in a] == [s.t for s in b] == outs
+    for sa, sb in zip(a, b):
+        for k in ("h", "u", "v", "eta1", "q1"):
+            assert np.array_equal(getattr(sa, k), getattr(sb, k)), (sa.t, k)
+        for k in sa.ledger:
+            assert np.array_equal(np.asarray(sa.ledger[k]), np.asarray(sb.ledger[k])), (sa.t, k)
