@@ -40,8 +40,8 @@ int launch_plaus(int dtype, const void* h, int64_t plane, int B, const his_net1d
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_finalize(const FinalizeArgs F) {
-    __shared__ double red[NT / 64][24];
-    __shared__ double tot[24];
+    __shared__ double red[NT / 64][FIN_NV];
+    __shared__ double tot[FIN_NV];
     finalize_member<NT>(F, blockIdx.x, red, tot);
 }
 
@@ -309,6 +309,10 @@ static int check_net(const his_net1d* n) {
         set_error("network descriptor has a NULL face array");
         return HIS_E_ARG;
     }
+    if (n->nparts > HIS_MAX_NET_PARTS) {
+        set_error("network descriptor splits into %d parts, more than %d", n->nparts, HIS_MAX_NET_PARTS);
+        return HIS_E_ARG;
+    }
     if (n->nparts > 1 && (!n->p_cptr || !n->p_cell || !n->p_fptr || !n->p_face || !n->p_gptr || !n->p_ghost)) {
         set_error("network descriptor splits into %d parts but a part array is NULL", n->nparts);
         return HIS_E_ARG;
@@ -479,6 +483,7 @@ static int engine_enqueue_step(his_engine* e, hipStream_t s, hipStream_t side, F
         d.inflow = G.inflow; d.stage = G.stage; d.acc = e->d.ctl.acc; d.active = e->active;
         d.rate = e->rate1d; d.wet = e->wet1d; d.Btot = e->Btot;
         d.member0 = m0;
+        d.error = e->d.ctl.error;
     }
     X.one_phase = e->ex_one;
     if (e->fuse_head) {

@@ -3,10 +3,11 @@
 // block per member, and the engine appends the same blocks to the next step's first exchange
 // launch so that the reduction needs no stream of its own.
 //
-// Each thread sums its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22) and rate maximum
-// (23); each quantity is then reduced by a fixed butterfly within every wave and the waves' results
-// are combined in wave order, so the result is bit-reproducible.  A depth or storage sum that is not
-// finite flags the member with error code 3.
+// Each thread sums its tiles' stage-1 sums (slots 0..8), stage-2 sums (10..22), rate maximum (23)
+// and depth maximum (24); each quantity is then reduced by a fixed butterfly within every wave and
+// the waves' results are combined in wave order, so the result is bit-reproducible.  A depth or
+// storage sum that is not finite flags the member with error code 3, a depth above the plausibility
+// limit with error code 4 (hydrointel/solver/coupling.py:169-192, checked here after every step).
 // Volumes follow hydrointel/solver/swe2d.py:_step_core 305-371: boundary volumes from the RK
 // average of the two stages' edge fluxes (324-328), rain (331-337), infiltration and clipping.
 #pragma once
@@ -19,9 +20,12 @@ __device__ __forceinline__ bool finite_bits(double x) {
     return ((unsigned long long)__double_as_longlong(x) & 0x7ff0000000000000ULL) != 0x7ff0000000000000ULL;
 }
 
+constexpr int FIN_NV = 25;      // reduced quantities per member: 9 + 13 sums (slot 9 unused), 2 maxima
+
 template <int NT>
-__device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int m, double (*red)[24], double* tot) {
-    constexpr int NV = 24;
+__device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int m, double (*red)[FIN_NV],
+                                                double* tot) {
+    constexpr int NV = FIN_NV;
     if (F.snap) {
         if (F.snap[2 * F.Btot + m] == 0.0) return;     // the member did not step
     } else if (F.active && !F.active[m]) {
@@ -42,12 +46,14 @@ __device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int
 #pragma unroll
             for (int k = 0; k <= P_NOUT; ++k) v[k] += p[k];
 #pragma unroll
-        for (int k = 0; k < P_USED - 1; ++k) v[10 + k] += p[NP + k];
+        for (int k = 0; k < P_NSUM; ++k) v[10 + k] += p[NP + k];
         v[23] = p[NP + P_RATE] > v[23] ? p[NP + P_RATE] : v[23];
+        v[24] = p[NP + P_HMAX] > v[24] ? p[NP + P_HMAX] : v[24];
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int k = 0; k < NV - 1; ++k) v[k] = wave_sum(v[k]);
+    for (int k = 0; k < NV - 2; ++k) v[k] = wave_sum(v[k]);
+    v[NV - 2] = wave_max(v[NV - 2]);
     v[NV - 1] = wave_max(v[NV - 1]);
     if (lane == 0)
 #pragma unroll
@@ -56,17 +62,18 @@ __device__ __forceinline__ void finalize_member(const FinalizeArgs& F, const int
     if (threadIdx.x < NV) {
         const int k = threadIdx.x;
         double a = red[0][k];
-        for (int w = 1; w < NT / 64; ++w) a = k == NV - 1 ? (red[w][k] > a ? red[w][k] : a) : a + red[w][k];
+        for (int w = 1; w < NT / 64; ++w) a = k >= NV - 2 ? (red[w][k] > a ? red[w][k] : a) : a + red[w][k];
         tot[k] = a;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double s1[P_NOUT + 1], s2[P_USED - 1];
+    double s1[P_NOUT + 1], s2[P_NSUM];
     for (int k = 0; k <= P_NOUT; ++k) s1[k] = tot[k];
-    for (int k = 0; k < P_USED - 1; ++k) s2[k] = tot[10 + k];
+    for (int k = 0; k < P_NSUM; ++k) s2[k] = tot[10 + k];
     const double rmax = tot[23];
     const double area = F.area[gi];
     if (F.error && F.error[m] == 0 && !(finite_bits(s2[P_VOLH]) && finite_bits(s2[P_VOLS]))) F.error[m] = 3;
+    if (F.error && F.error[m] == 0 && !(tot[24] <= HIS_PLAUSIBLE_DEPTH_2D_M)) F.error[m] = 4;
     if (F.rate) F.rate[m] = rmax;
     if (F.volume) {
         F.volume[m] = s2[P_VOLH] * area;

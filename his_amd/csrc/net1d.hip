@@ -108,10 +108,11 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
     double *y = ws, *u = ws + nc, *th = ws + 2 * nc;
     double *F1 = ws + 3 * nc, *F2 = F1 + nf, *pL = F2 + nf, *pR = pL + nf;
     const double g = N.g, h_dry = N.h_dry;
-    // split networks: this part's ledger partials (b_in, b_out, clip, rate, wet) and the member's counter
-    double* led = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + ((int64_t)b * P + p) * 5;
+    // split networks: this part's ledger partials (b_in, b_out, clip, rate, and wet + 64 x implausible:
+    // summed over at most 16 parts, the sum still tells both flags apart) and the member's counter
+    double* led = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + ((int64_t)b * P + p) * LED_PART;
     unsigned int* cnt = reinterpret_cast<unsigned int*>(wsg + (int64_t)N.B * (3 * nc + 4 * nf) +
-                                                        (int64_t)N.B * P * 5 + b);
+                                                        (int64_t)N.B * P * LED_PART + b);
     double led_in = 0.0, led_out = 0.0, led_clip = 0.0;
     int nsub = 1;
     double dt_full = d.dt[m], t0 = 0.0;
@@ -138,6 +139,12 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
     // CFL rate of the final state (swe1d.py:253-262), formed in the last substep's update
     const bool want_rate = d.rate || d.wet;
     double r_mx = 0.0, r_wet = 0.0;
+    // plausibility guard of the final state (coupling.py:169-192): a channel level more than
+    // HIS_PLAUSIBLE_RISE_1D_M above its bank crest, or not finite, flags the member (error code 5).
+    // The crest is taken as bed + bankfull depth, which equals it wherever the bank stands 0.2 m or
+    // more above the bed (the host's check at check times uses the crest itself)
+    const bool guard = d.engine && d.error;
+    double implaus = 0.0;
     for (int k = 0; k < nsub; ++k) {
         double q_in, eta;
         if (d.engine) {
@@ -274,6 +281,9 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
             if (!((y1 >= h_dry) && kd != 1)) Q1 = 0.0;
             A[c] = A1;
             Q[c] = Q1;
+            // level above the bank crest: the depth above the section's bankfull depth, which is
+            // the crest above the bed (at least 0.2 m), so no further loads are needed
+            if (guard && k == nsub - 1 && !(y1 - s.yb <= HIS_PLAUSIBLE_RISE_1D_M)) implaus = 1.0;
             if (want_rate && k == nsub - 1 && y1 > h_dry) {
                 r_wet = 1.0;
                 const double cc = sqrt(g * A1 / sec_top(s, y1));
@@ -301,40 +311,46 @@ __device__ void swe1d_member(const his_net1d& N, double* __restrict__ Ag, double
         }
         __syncthreads();
     }
-    double rate_blk = 0.0, wet_blk = 0.0;
-    if (want_rate) {
+    double rate_blk = 0.0, wet_blk = 0.0, bad_blk = 0.0;
+    if (want_rate || guard) {
         r_mx = wave_max(r_mx);
         r_wet = wave_max(r_wet);
+        implaus = wave_max(implaus);
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        if (lane == 0) { red[w][0] = r_mx; red[w][1] = r_wet; }
+        if (lane == 0) { red[w][0] = r_mx; red[w][1] = r_wet; red[w][2] = implaus; }
         __syncthreads();
         if (threadIdx.x == 0) {
             for (int w2 = 0; w2 < NT / 64; ++w2) {
                 rate_blk = red[w2][0] > rate_blk ? red[w2][0] : rate_blk;
                 wet_blk = red[w2][1] > wet_blk ? red[w2][1] : wet_blk;
+                bad_blk = red[w2][2] > bad_blk ? red[w2][2] : bad_blk;
             }
             if (P == 1) {
-                if (d.rate) d.rate[m] = rate_blk;
-                if (d.wet) d.wet[m] = wet_blk > 0.0 ? 1 : 0;
+                if (want_rate && d.rate) d.rate[m] = rate_blk;
+                if (want_rate && d.wet) d.wet[m] = wet_blk > 0.0 ? 1 : 0;
+                if (guard && bad_blk > 0.0 && d.error[m] == 0) d.error[m] = 5;
             }
         }
     }
     if (P > 1 && threadIdx.x == 0) {
-        led[0] = led_in; led[1] = led_out; led[2] = led_clip; led[3] = rate_blk; led[4] = wet_blk;
+        led[0] = led_in; led[1] = led_out; led[2] = led_clip; led[3] = rate_blk;
+        led[4] = (wet_blk > 0.0 ? 1.0 : 0.0) + (bad_blk > 0.0 ? 64.0 : 0.0);
         __threadfence();
         const unsigned int done = atomicAdd(cnt, 1u);
         if (done == (unsigned int)(P - 1)) {
             // every other part has published its partials (each fenced before its increment)
             __threadfence();
-            const volatile double* all = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + (int64_t)b * P * 5;
-            double s_in = 0.0, s_out = 0.0, s_clip = 0.0, s_rate = 0.0, s_wet = 0.0;
+            const volatile double* all = wsg + (int64_t)N.B * (3 * nc + 4 * nf) + (int64_t)b * P * LED_PART;
+            double s_in = 0.0, s_out = 0.0, s_clip = 0.0, s_rate = 0.0, s_flags = 0.0;
             for (int q = 0; q < P; ++q) {
-                s_in += all[q * 5 + 0];
-                s_out += all[q * 5 + 1];
-                s_clip += all[q * 5 + 2];
-                s_rate = all[q * 5 + 3] > s_rate ? all[q * 5 + 3] : s_rate;
-                s_wet = all[q * 5 + 4] > s_wet ? all[q * 5 + 4] : s_wet;
+                s_in += all[q * LED_PART + 0];
+                s_out += all[q * LED_PART + 1];
+                s_clip += all[q * LED_PART + 2];
+                s_rate = all[q * LED_PART + 3] > s_rate ? all[q * LED_PART + 3] : s_rate;
+                s_flags += all[q * LED_PART + 4];
             }
+            const double s_wet = s_flags >= 64.0 ? s_flags - 64.0 * floor(s_flags / 64.0) : s_flags;
+            if (guard && s_flags >= 64.0 && d.error[m] == 0) d.error[m] = 5;
             if (d.engine) {
                 d.acc[4 * d.Btot + m] += s_in;
                 d.acc[5 * d.Btot + m] += s_out;
@@ -396,16 +412,16 @@ __global__ void __launch_bounds__(NT) k_swe1d_rate(const his_net1d N, const doub
 }
 
 // per member: y, u, theta (nc each), F1, F2, pL, pR (nf each); then for a split network the parts'
-// ledger partials (5 per part) and one counter slot per member
+// ledger partials (LED_PART per part) and one counter slot per member
 size_t swe1d_ws_doubles(const his_net1d& n) {
     const size_t P = n.nparts > 1 ? (size_t)n.nparts : 1;
-    const size_t split = n.nparts > 1 ? 5 * P + 1 : 0;
+    const size_t split = n.nparts > 1 ? LED_PART * P + 1 : 0;
     return (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf + split);
 }
 
 unsigned int* swe1d_counters(const his_net1d& n, double* ws) {
     return reinterpret_cast<unsigned int*>(ws + (size_t)n.B * (3 * (size_t)n.nc + 4 * (size_t)n.nf) +
-                                           (size_t)n.B * 5 * (size_t)n.nparts);
+                                           (size_t)n.B * LED_PART * (size_t)n.nparts);
 }
 
 int clear_swe1d_counters(const his_net1d& n, double* ws, hipStream_t s) {
@@ -564,8 +580,8 @@ template <typename R2>
 __global__ void __launch_bounds__(256) k_ex_links(const ExGroups G) {
     __shared__ double red[4][3];
     __shared__ HeadState hs[2];
-    __shared__ double fred[4][24];
-    __shared__ double ftot[24];
+    __shared__ double fred[4][FIN_NV];
+    __shared__ double ftot[FIN_NV];
     const int64_t total = G.it_links[G.ngroups];
     // the previous step's ledger reduction takes the first blocks, so it is dispatched first and
     // overlaps the link blocks

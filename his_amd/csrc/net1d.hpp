@@ -27,7 +27,15 @@ struct Net1DDrive {
     int ex_nl, ex_stride;
     double *t_out;
     int32_t *steps;
+    // engine mode, or NULL: a member whose final channel state fails the plausibility guard gets
+    // error code 5 (unless it carries one already)
+    int32_t *error;
 };
+
+// doubles per part of a split network's ledger partials: b_in, b_out, clip, rate, flags (wet +
+// 64 x implausible, summed over the parts: HIS_MAX_NET_PARTS < 64 keeps the two apart)
+constexpr int LED_PART = 5;
+static_assert(HIS_MAX_NET_PARTS < 64, "the parts' flag sum needs fewer than 64 parts");
 
 // one workgroup per (member, part); width = the most cells or faces of any part (swe1d_part_width)
 int launch_swe1d(const his_net1d& n, double* A, double* Q, const Net1DDrive& d, double* ws, int width,

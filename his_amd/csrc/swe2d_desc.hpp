@@ -13,8 +13,10 @@ enum {
     P_RAINW,                    // rain depth added (rain_weight runs only)
     P_VOLH, P_VOLS,             // sum of h, sum of s_filled (new state)
     P_RATE,                     // CFL rate maximum (new state)
+    P_HMAX,                     // largest depth of the new state (the plausibility guard)
     P_USED
 };
+constexpr int P_NSUM = P_RATE;  // stage-2 slots that are sums (the rest are maxima)
 constexpr int NP = 16;          // slots per stage block
 constexpr int NP2 = 2 * NP;     // slots per tile
 

@@ -30,6 +30,15 @@ extern "C" {
 #define HIS_BC_STAGE 2
 #define HIS_BC_DISCHARGE 3
 
+/* Plausibility limits the engine checks on the device after every coupled step, as hydrointel's
+ * check_plausible does at its check times (hydrointel/solver/coupling.py:169-192): the deepest 2-D
+ * cell, and the highest channel level above its bank crest (interior 1-D cells), in metres. */
+#define HIS_PLAUSIBLE_DEPTH_2D_M 10.0
+#define HIS_PLAUSIBLE_RISE_1D_M 10.0
+
+/* Most parts a network may be split into (his_net1d.nparts). */
+#define HIS_MAX_NET_PARTS 63
+
 #define HIS_E_ARG (-1)
 #define HIS_E_HIP (-2)
 #define HIS_E_WORKSPACE (-3)
@@ -124,7 +133,7 @@ typedef struct his_net1d {
      * components), which separate workgroups then step without synchronising: part p owns the cells
      * p_cell[p_cptr[p] .. p_cptr[p+1]), the faces p_face[p_fptr[p] .. p_fptr[p+1]) and the ghosts
      * p_ghost[p_gptr[p] .. p_gptr[p+1]) (indices into g_idx), each list ascending.  nparts <= 1 =
-     * one part, the whole network (the arrays may then be NULL). */
+     * one part, the whole network (the arrays may then be NULL); at most HIS_MAX_NET_PARTS. */
     int32_t nparts;
     int32_t part_width;                    /* the most cells or faces of any part: sizes the workgroup */
     const int32_t *p_cptr, *p_cell, *p_fptr, *p_face, *p_gptr, *p_ghost;
@@ -217,8 +226,11 @@ typedef struct his_control {
     int32_t *n_sub_max; /* (Btot) */
     int32_t *error;     /* (Btot) 0 ok; 1 = 1-D substeps above n_sub_limit, 2 = non-finite CFL rate,
                            3 = non-finite 2-D depth or storage (found by the step's float64 sums; a
-                           non-finite momentum reaches the depth through the next step's fluxes): the
-                           member stops (like hydrointel's NumericalInstabilityError, coupling.py:169-192) */
+                           non-finite momentum reaches the depth through the next step's fluxes),
+                           4 = a 2-D depth above HIS_PLAUSIBLE_DEPTH_2D_M after the step,
+                           5 = a channel level more than HIS_PLAUSIBLE_RISE_1D_M above its bank (or not
+                           finite) after the step: the member stops (like hydrointel's
+                           NumericalInstabilityError and check_plausible, coupling.py:169-192) */
     int32_t n_sub_limit;
 } his_control;
 

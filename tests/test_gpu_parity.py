@@ -643,3 +643,47 @@ def test_async_outputs_equal_sync_outputs(gpu):
             assert np.array_equal(getattr(sa, k), getattr(sb, k)), (sa.t, k)
         for k in sa.ledger:
             assert np.array_equal(np.asarray(sa.ledger[k]), np.asarray(sb.ledger[k])), (sa.t, k)
+
+
+def test_device_guard_flags_an_implausible_2d_depth(gpu):
+    """A 2-D depth above 10 m after a step flags that member on the device (error code 4), and the
+    run then raises, naming the member; the other member is not flagged."""
+    from his_amd.solver.coupling import NumericalInstabilityError
+    eng, _ = coupled_case(gpu)
+    two = eng.two
+    j, i = two.ny // 2, two.nx // 2
+    two.h[1, j, i] = 11.0 + float(two.h[1, j, i])     # a column of water the step cannot drain
+    dev = eng._device()
+    dev.refresh()
+    dev.t.fill_(0.0)
+    dev.t_end.fill_(1e12)
+    dev.t_limit.fill_(1e12)
+    dev.step(1)
+    assert dev.error.cpu().tolist() == [0, 4]
+    with pytest.raises(NumericalInstabilityError, match="member 1"):
+        eng._check_device_errors(dev, float(dev.t[0].item()), "guard")
+
+
+def test_device_guard_flags_an_implausible_channel_level(gpu):
+    """A channel level more than 10 m above its bank after a step flags that member on the device
+    (error code 5); the other member is not flagged.  The weir coefficient is zero here so that the
+    exchange, which would otherwise spill the excess onto the floodplain, leaves it in the channel."""
+    from his_amd import workloads as W
+    from his_amd.config import SolverConfig
+    from his_amd.engine import build_from_workload
+    wl = W.plain_cpu()
+    wl.members = [wl.members[0], wl.members[0]]
+    eng, _ = build_from_workload(wl, SolverConfig(weir_cw=0.0), gpu, batched=True)
+    one = eng.one
+    c = int(np.flatnonzero(one.topo.kind <= 1)[len(one.topo.kind) // 3])
+    ys = one.sec.depth(one.A).clone()
+    ys[1, c] = float(one.topo.zbank[c]) + 40.0 - float(one.z[1, c])
+    one.A[1, c] = one.sec.area(ys)[1, c]
+    dev = eng._device()
+    dev.refresh()
+    dev.t.fill_(0.0)
+    dev.t_end.fill_(1e12)
+    dev.t_limit.fill_(1e12)
+    dev.step(1)
+    assert dev.error.cpu().tolist() == [0, 5]
+
