@@ -117,14 +117,22 @@ def _run_to(eng, t_end, interval=900.0):
 def test_precision_form_fp32_bulk_tracks_fp64(gpu):
     """400 x 325, B = 32, float32 bulk with float64 accumulators, first 3 h of the storm (rain
     rising to its peak), against the same 32 members in float64: peak depth within 1 cm for
-    every member (the 1 cm bar), volumes within 1e-3 relative, every member mass-balanced."""
+    every member (the 1 cm bar), volumes within 1e-3 relative, every member mass-balanced.  The
+    float32 run uses the kernel shape the benchmark's coupled-step tuner picks for this workload
+    (58-column strips x 19-row chunks)."""
+    from his_amd import _lib
     from his_amd import workloads as W
     from his_amd.engine import build_from_workload
     wl = W.precision_form()
     T = 3 * 3600.0
     e32, _ = build_from_workload(wl, device=gpu)
     assert e32.two.dtype == torch.float32 and e32.two.B == 32
-    p32, v32, m32 = _run_to(e32, T)
+    try:
+        _lib.set_tile_shape(torch.float32, 1)
+        assert _lib.tile_shape(torch.float32, 1)[:2] == (58, 19)
+        p32, v32, m32 = _run_to(e32, T)
+    finally:
+        _lib.set_tile_shape(torch.float32, -1)
     steps32 = e32.log.steps
     del e32
     wl64 = W.precision_form()
@@ -140,11 +148,16 @@ def test_precision_form_fp32_bulk_tracks_fp64(gpu):
     assert m32.max() < 1e-3 and m64.max() < 1e-3, (m32.max(), m64.max())
 
 
-def test_precision_form_members_track_the_oracle(gpu):
+# float32 kernel shapes: the default (58-column strips x 8-row chunks, last chunk 5 rows at 325 rows)
+# and the two the coupled-step tuner picks for this workload (19 rows: a last chunk of 2 rows; 12
+# rows: 1 row)
+@pytest.mark.parametrize("shape", [2, 1, 7])
+def test_precision_form_members_track_the_oracle(gpu, shape):
     """Two float32-bulk members of the 400 x 325 workload, wet after 1 h, take 5 coupled steps
     beside the float64 oracle under the same dt and n_sub: every cell of h, hu, hv within
     float32 round-off of the state's scale (5e-5 of max |value|, plus 1e-6), A and Q within 1e-6
     relative of their scale."""
+    from his_amd import _lib
     from his_amd import workloads as W
     from his_amd.engine import build_from_workload
     wl = W.precision_form()
@@ -154,12 +167,16 @@ def test_precision_form_members_track_the_oracle(gpu):
     eng.run(t, out_times=[])
     orc = _oracle_for(wl, [0, 1], wl.slot_width_frac)
     _load_oracle(orc, eng)
-    for s in range(5):
-        dt = eng.two.max_dt()
-        n_sub = max(1, math.ceil(dt / eng.one.max_dt() - 1e-9))
-        t_dev = eng.step(t, 1e12, dt=dt, n_sub=n_sub)
-        t = orc.step(t, 1e12, dt=dt, n_sub=n_sub)
-        assert t_dev == t
+    try:
+        _lib.set_tile_shape(torch.float32, shape)
+        for s in range(5):
+            dt = eng.two.max_dt()
+            n_sub = max(1, math.ceil(dt / eng.one.max_dt() - 1e-9))
+            t_dev = eng.step(t, 1e12, dt=dt, n_sub=n_sub)
+            t = orc.step(t, 1e12, dt=dt, n_sub=n_sub)
+            assert t_dev == t
+    finally:
+        _lib.set_tile_shape(torch.float32, -1)
     for name, ref in (("h", orc.h), ("hu", orc.hu), ("hv", orc.hv), ("s_filled", orc.s)):
         got = _f64(getattr(eng.two, name))
         scale = float(np.abs(ref).max())
