@@ -189,12 +189,14 @@ def index_entry(path: Path, res, sea: np.ndarray, meta: dict, f: int, wall_s: fl
             "in_distribution": bool(res.in_distribution)}
 
 
-def horton_rate_at(landuse, kappa, times, f: int) -> np.ndarray:
-    """Horton capacity f(t) = fc + (f0 - fc) exp(-k t) [mm/h] of the land use scaled by kappa, at each
-    output time, block-averaged onto the storage grid: (nt, ny/f, nx/f)."""
+def horton_rate_at(landuse, kappa, times, f: int, storm_start: float = 0.0) -> np.ndarray:
+    """Horton capacity f(t) = fc + (f0 - fc) exp(-k t) [mm/h] of the land use scaled by kappa, with t
+    the time since the storm started (0 before it), at each output time, block-averaged onto the
+    storage grid: (nt, ny/f, nx/f) (generate.py:400-406)."""
     from .domain import landuse as LU
     f0, fc, k = LU.horton_fields(landuse, kappa)
-    out = [coarsen_mean((fc + (f0 - fc) * np.exp(-k * float(t))) / MMH, f) for t in np.asarray(times)]
+    out = [coarsen_mean((fc + (f0 - fc) * np.exp(-k * max(float(t) - storm_start, 0.0))) / MMH, f)
+           for t in np.asarray(times)]
     return np.stack(out).astype(np.float32)
 
 
@@ -215,7 +217,7 @@ def write_storms(out, storms: list, f: int = 1, compress: bool = False, forcing_
     A batch of one that the engine cannot simulate is recorded as failed; in a larger batch the
     failing member cannot be told apart and the error propagates."""
     from . import api
-    from .scenarios import sea_level_rise, rain_factor
+    from .scenarios import rain_factor, resolve, sea_level_rise
     from .solver.coupling import NumericalInstabilityError
     ctx = api.context()
     dom = ctx.domain
@@ -244,15 +246,19 @@ def write_storms(out, storms: list, f: int = 1, compress: bool = False, forcing_
     for (i, site, scen, meta), res in zip(run, results):
         path = record_path(out, i, compress)
         lu = ctx.landuse(scen.ssp, scen.horizon_year)
+        fo = res.extras["forcing"]
+        # the areal reduction factor the resolver applied to this storm's rain (1 without one)
+        arf = getattr(resolve(scen, dom), "arf_domain", 1.0)
         meta = {**meta, "return_period_yr": scen.return_period_yr, "rcp": scen.rcp, "ssp": scen.ssp,
                 "horizon_year": scen.horizon_year, "tide_on": scen.tide_on,
                 "storm_tide_offset_h": scen.storm_tide_offset_h}
         rec = simulation_record(res, f, dom.dx, site=site, meta=meta, landuse_frac=landuse_fractions(lu),
                                 horton_rate_mmh=horton_rate_at(lu, site.infil_multiplier.double().numpy(),
-                                                               res.times.numpy(), f),
+                                                               res.times.numpy(), f, float(fo.storm_start)),
                                 forcing_steps=forcing_steps, wall_s=wall,
                                 forcing_factors={"rain_factor": rain_factor(scen.rcp),
-                                                 "slr_m": sea_level_rise(scen.rcp, scen.horizon_year)})
+                                                 "slr_m": sea_level_rise(scen.rcp, scen.horizon_year),
+                                                 "arf": arf})
         save_record(rec, path)
         index[str(i)] = index_entry(path, res, dom.sea, meta, f, wall, batch_wall, len(run))
         index_path.write_text(json.dumps(index, indent=1), encoding="utf-8")
