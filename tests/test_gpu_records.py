@@ -52,6 +52,8 @@ def test_write_storms_and_read_back(gpu, tmp_path):
             assert rec["fields"]["landuse_frac"].shape == (8, dom.ny // 2, dom.nx // 2)
             assert torch.allclose(rec["fields"]["landuse_frac"].sum(0), torch.ones(dom.ny // 2, dom.nx // 2))
             assert rec["forcing"]["horton_rate_mmh"].shape == (nt, dom.ny // 2, dom.nx // 2)
+            # this package's design storm applies no areal reduction; hydrointel's would carry its own
+            assert rec["forcing"]["arf"] == 1.0 and rec["forcing"]["storm_start"] == 0.0
             e = index[str(i)]
             assert e["record"] == f"sim_{i:05d}.pt" and e["batch_size"] == 2 and e["name"] == storms[i][3]["name"]
             land = ~dom.sea
