@@ -190,10 +190,6 @@ def check_site(site: SiteState, scenario: Scenario | None = None):
 
 
 # -------------------------------------------------------------------------- runs
-def _horton(lu, kappa):
-    return LU.horton_fields(lu, kappa)
-
-
 def _snapshots(eng, t_end: float, times, label_: str):
     snaps = []
     # the snapshots are only collected here, so their arrays may reach the host while the run steps on

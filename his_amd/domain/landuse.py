@@ -52,12 +52,6 @@ def horton_fields(raster, kappa=1.0):
             lookup(raster, "k_per_h") / 3600.0)
 
 
-def class_fractions(raster) -> np.ndarray:
-    """Share of the cells in each class, (N_CLASSES,)."""
-    codes = np.asarray(raster).astype(np.int64).reshape(-1)
-    return np.bincount(codes, minlength=N_CLASSES)[:N_CLASSES] / max(codes.size, 1)
-
-
 def synthetic_landuse(terrain, channel_mask, seed: int = 7) -> np.ndarray:
     """A class map for a coastal-plain terrain whose sea lies along its last columns (int8, (ny, nx))."""
     ny, nx = terrain.shape

@@ -133,21 +133,6 @@ def triangle(t_start: float, duration: float, peak, base=0.0, batched: bool = Tr
     return PiecewiseLinear(t_start, 0.5 * duration, np.stack([bs, pk, bs], axis=1), batched=batched)
 
 
-def select(series: Series, idx) -> Series:
-    """The same series restricted to the members ``idx`` (for one request out of a batch)."""
-    import copy
-    idx = np.atleast_1d(np.asarray(idx))
-    out = copy.deepcopy(series)
-    if isinstance(series, (PiecewiseLinear, PiecewiseConstant)):
-        out.values = series.values[idx].copy()
-    elif isinstance(series, Harmonic):
-        out.base, out.shift, out.on = series.base[idx].copy(), series.shift[idx].copy(), series.on[idx].copy()
-    else:
-        out.values = series.values[idx].copy()
-    out.n = idx.size
-    return out
-
-
 def stack(members: list) -> Series:
     """One batched series from one-member series of the same form on a shared time base (the same
     knots, or the same tidal constituents): hydrointel's BatchForcing enforces a shared time base too

@@ -135,12 +135,6 @@ def tune_coupled(eng, steps: int = 60, warm: int = 4) -> tuple[int, dict]:
     return best, times
 
 
-def timings(two) -> dict:
-    """The remembered per-shape microseconds of this solver's grid ({} if it was never timed)."""
-    rec = _load().get(key(two.dtype, two.nx, two.ny, two.B), {})
-    return {int(k): v for k, v in rec.get("all_us", {}).items()}
-
-
 def describe(dtype, shape: int) -> str:
     tx, ty, var = L.tile_shape(dtype, shape)
     return f"{('tile', 'march', 'wave')[var]} {tx}x{ty}"
