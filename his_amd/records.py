@@ -217,7 +217,7 @@ def write_storms(out, storms: list, f: int = 1, compress: bool = False, forcing_
     A batch of one that the engine cannot simulate is recorded as failed; in a larger batch the
     failing member cannot be told apart and the error propagates."""
     from . import api
-    from .scenarios import rain_factor, resolve, sea_level_rise
+    from .scenarios import rain_factor, sea_level_rise
     from .solver.coupling import NumericalInstabilityError
     ctx = api.context()
     dom = ctx.domain
@@ -247,18 +247,18 @@ def write_storms(out, storms: list, f: int = 1, compress: bool = False, forcing_
         path = record_path(out, i, compress)
         lu = ctx.landuse(scen.ssp, scen.horizon_year)
         fo = res.extras["forcing"]
-        # the areal reduction factor the resolver applied to this storm's rain (1 without one)
-        arf = getattr(resolve(scen, dom), "arf_domain", 1.0)
+        # the design factors of this storm's forcing (generate.py:383-384) when its resolver records
+        # them, else the scenario's pathway values and no areal reduction (the default design storm)
+        factors = {"rain_factor": getattr(fo, "rain_factor", rain_factor(scen.rcp)),
+                   "slr_m": getattr(fo, "slr_m", sea_level_rise(scen.rcp, scen.horizon_year)),
+                   "arf": getattr(fo, "arf_domain", 1.0)}
         meta = {**meta, "return_period_yr": scen.return_period_yr, "rcp": scen.rcp, "ssp": scen.ssp,
                 "horizon_year": scen.horizon_year, "tide_on": scen.tide_on,
                 "storm_tide_offset_h": scen.storm_tide_offset_h}
         rec = simulation_record(res, f, dom.dx, site=site, meta=meta, landuse_frac=landuse_fractions(lu),
                                 horton_rate_mmh=horton_rate_at(lu, site.infil_multiplier.double().numpy(),
                                                                res.times.numpy(), f, float(fo.storm_start)),
-                                forcing_steps=forcing_steps, wall_s=wall,
-                                forcing_factors={"rain_factor": rain_factor(scen.rcp),
-                                                 "slr_m": sea_level_rise(scen.rcp, scen.horizon_year),
-                                                 "arf": arf})
+                                forcing_steps=forcing_steps, wall_s=wall, forcing_factors=factors)
         save_record(rec, path)
         index[str(i)] = index_entry(path, res, dom.sea, meta, f, wall, batch_wall, len(run))
         index_path.write_text(json.dumps(index, indent=1), encoding="utf-8")
