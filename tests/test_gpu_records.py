@@ -9,6 +9,7 @@ import torch
 
 from his_amd import api
 from his_amd import records as R
+from his_amd import scenarios as S
 
 pytestmark = pytest.mark.gpu
 
@@ -68,5 +69,18 @@ def test_write_storms_and_read_back(gpu, tmp_path):
         rec2 = R.load_record(R.record_path(tmp_path, 2))
         assert R.record_path(tmp_path, 2).name == "sim_00002.pt.gz" and index["2"]["batch_size"] == 1
         assert tuple(rec2["h"].shape) == (nt, dom.ny, dom.nx)
+        # a resolver whose forcing records its design factors, as hydrointel's does, has them written
+        # (generate.py:383-384) in place of the pathway values
+        design = S.DesignStorm()
+
+        def with_factors(s, domain=None):
+            fo = design(s, domain)
+            fo.rain_factor, fo.slr_m, fo.arf_domain = 1.25, 0.5, 0.93
+            return fo
+        S.register_resolver(with_factors)
+        R.write_storms(tmp_path, [(3, site, sc[0], {"name": "factors"})], f=2)
+        got = R.load_record(R.record_path(tmp_path, 3))["forcing"]
+        assert (got["rain_factor"], got["slr_m"], got["arf"]) == (1.25, 0.5, 0.93)
     finally:
+        S.register_resolver(S.DesignStorm())
         api.configure()
