@@ -11,9 +11,10 @@ length. Here:
   one coupled step; idle slots return at the top of each kernel.
 * Admission: an arriving request waits in its size's queue until a slot is free. It is
   admitted only if the HBM footprint of all busy slots plus its own stays within the
-  budget. Its state, fields, channel sections and forcing rows are copied into the slot
-  from a one-member engine built exactly as a standalone run builds it. Only that
-  slot's CFL rates are recomputed (``his_engine_refresh_members``).
+  budget. Its state, fields, channel sections and forcing rows are written into the slot
+  straight from its member draw, with the operations a standalone build of the same storm
+  performs (``Pool.load_request``); no engine is built per request. Only that slot's CFL
+  rates are recomputed (``his_engine_refresh_members``).
 * A request's result is therefore the result of the same storm run alone, bit for bit:
   every kernel computes a member from that member's data only, and partial sums are
   reduced per member in a fixed order. ``tests/test_gpu_parity.py`` checks this.
@@ -259,28 +260,6 @@ class Pool:
         """{slot: (total seaward channel discharge, river mouth's)} for the given slots, one copy."""
         Q = self.eng.one.Q.detach().cpu().numpy()
         return {b: sea_discharge(Q[b], self.out_cells, self.out_signs, self.out_main) for b in members}
-
-    def load(self, b: int, req_eng: CoupledEngine):
-        """Copy a one-member engine's state, fields and sections into slot b (in place)."""
-        two, one = self.eng.two, self.eng.one
-        r2, r1 = req_eng.two, req_eng.one
-        for name in ("h", "hu", "hv", "s_filled", "infil_cum"):
-            getattr(two, name)[b].copy_(getattr(r2, name)[0])
-        two.n[b].copy_(r2.n[0])
-        two.storage[b].copy_(r2.storage[0])
-        for name in ("f0", "fc", "k"):
-            getattr(two.horton, name)[b].copy_(getattr(r2.horton, name)[0])
-        one.A[b].copy_(r1.A[0])
-        one.Q[b].copy_(r1.Q[0])
-        one.z[b].copy_(r1.z[0])
-        for sec_p, sec_r in ((one.sec, r1.sec), (one.fsec, r1.fsec)):
-            for name in ("b", "yb", "ts"):
-                dst, src = getattr(sec_p, name), getattr(sec_r, name)
-                if dst.dim() == 2:
-                    dst[b].copy_(src[0])
-                elif not torch.equal(dst, src.reshape(dst.shape)):
-                    raise ValueError(f"section field {name} is shared in the pool but differs for the request")
-        self.hmax[b].copy_(two.h[b])
 
 
 @dataclass

@@ -26,9 +26,14 @@ class Horton:
         return cls(r, r.clone(), torch.zeros(shape, device=device, dtype=dtype))
 
     @classmethod
-    def from_mmh(cls, f0_mmh, fc_mmh, k_per_h, device, dtype, t0: float = 0.0) -> "Horton":
+    def from_landuse(cls, landuse, kappa, device, dtype, t0: float = 0.0) -> "Horton":
+        """The land-use classes' capacities times kappa and their decay rates
+        (hydrointel/solver/infiltration.py:23-28), formed in float64 and stored in ``dtype``."""
+        from ..domain.landuse import horton_fields
+        kap = kappa.detach().cpu().numpy() if torch.is_tensor(kappa) else kappa
         t = lambda a: torch.as_tensor(a, dtype=dtype, device=device)
-        return cls(t(f0_mmh) * MMH_TO_MS, t(fc_mmh) * MMH_TO_MS, t(k_per_h) / 3600.0, t0)
+        f0, fc, k = horton_fields(landuse, kap)
+        return cls(t(f0), t(fc), t(k), t0)
 
     def rate(self, t: float) -> torch.Tensor:
         tt = torch.clamp(torch.as_tensor(t) - self.t0, min=0.0)

@@ -60,10 +60,6 @@ class MassBalance:
             raise KeyError(name)
         self.ledger[_ROW[name]] += self._column(name, value)
 
-    def add_many(self, terms: dict) -> None:
-        for name, value in terms.items():
-            self.add(name, value)
-
     def fold_engine_rows(self, acc: torch.Tensor) -> None:
         """Add the device engine's ledger rows for these members, acc = (10, B) float64: rain,
         infiltration, bnd2d_in, bnd2d_out, bnd1d_in, bnd1d_out, clip2d, clip1d, exchanged in/out."""

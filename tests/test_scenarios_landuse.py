@@ -1,7 +1,8 @@
 """Land use, scenarios and the run-level API's host logic (no GPU needed).
 
 * Horton and roughness fields of a mixed-class raster equal the class table's values, the capacities
-  scaled by kappa and the decay not (hand-written expected values per class);
+  scaled by kappa and the decay not (hand-written expected values per class), also as
+  ``Horton.from_landuse`` builds them, whose rate decays from its clock start;
 * scenario validation raises ValueError like hydrointel's; the default design storm peaks at the
   return period's intensity times the RCP's rain factor at mid-storm, has high water at the requested
   offset, and holds the sea level of the RCP and horizon year when the tide is off;
@@ -31,6 +32,12 @@ def test_fields_of_a_mixed_class_raster():
     want_k = np.array([[0.0, 0.0, 4.0, 4.0], [3.0, 3.0, 2.0, 2.0]]) / 3600.0
     assert np.allclose(f0, want_f0, rtol=1e-15, atol=0) and np.allclose(fc, want_fc, rtol=1e-15, atol=0)
     assert np.allclose(k, want_k, rtol=1e-15, atol=0)
+    from his_amd.solver.infiltration import Horton
+    hz = Horton.from_landuse(raster, torch.as_tensor(kappa), "cpu", torch.float64, t0=600.0)
+    assert np.array_equal(hz.f0.numpy(), f0) and np.array_equal(hz.fc.numpy(), fc) and np.array_equal(hz.k.numpy(), k)
+    assert hz.t0 == 600.0 and torch.equal(hz.rate(600.0), hz.f0) and torch.equal(hz.rate(0.0), hz.f0)
+    late = hz.rate(600.0 + 3600.0).numpy()
+    assert np.allclose(late, want_fc + (want_f0 - want_fc) * np.exp(-want_k * 3600.0), rtol=1e-14, atol=0)
     assert LU.lookup(raster, "manning").tolist() == [[0.030, 0.014, 0.050, 0.045], [0.025, 0.035, 0.040, 0.120]]
     assert LU.lookup(raster, "storage_m")[1, 2] == 0.05
     with pytest.raises(ValueError):
