@@ -123,15 +123,17 @@ def context() -> Context:
     return _CTX[0] or configure()
 
 
-def grown_landuse(domain, ssp: str, year: int) -> np.ndarray:
-    """The domain's land use at a horizon year under an SSP: cropland, bare ground and grass turn
-    residential (dense urban nearest the core) at the pathway's expansion rate per decade since 2020,
-    nearest the existing town first.  Deterministic."""
-    from .scenarios import SSP_TABLE
+def grown_landuse(domain, ssp, year: int, base_year: int = 2025) -> np.ndarray:
+    """The domain's land use at a horizon year under an SSP (a name or an ``SSPState``): cropland, bare
+    ground and grass turn residential (dense urban nearest the core) at the pathway's expansion rate
+    per decade since ``base_year`` (hydrointel's origin, engine.py:73-75), nearest the existing town
+    first.  Deterministic.  The growth rule is this package's; hydrointel's own expansion model sits
+    upstream of the engine."""
+    from .scenarios import ssp_state
     base = np.asarray(domain.landuse)
-    rate = SSP_TABLE[ssp][0]
+    rate = (ssp_state(ssp) if isinstance(ssp, str) else ssp).urban_expansion_rate
     grow = np.isin(base, (LU.CROPLAND, LU.BARE, LU.GRASS)) & ~domain.channel & ~domain.sea
-    n_new = int(round(rate * max(year - 2020, 0) / 10.0 * grow.sum()))
+    n_new = int(round(rate * max(year - base_year, 0) / 10.0 * grow.sum()))
     if n_new <= 0:
         return base.copy()
     from scipy.ndimage import distance_transform_edt

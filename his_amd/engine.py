@@ -61,6 +61,30 @@ class Forcing:
                        stack([m.tide for m in members]), f0.t_end, f0.storm_start, None)
 
 
+def BatchForcing(forcings: list) -> Forcing:  # noqa: N802 -- hydrointel's class name
+    """B members' forcing on one shared time base, as hydrointel's ``BatchForcing(forcings)``
+    (engine.py:104-143) builds it: ``Forcing.stack``; ValueError when the members differ in it."""
+    return Forcing.stack(list(forcings))
+
+
+def build_forcing(cfg, return_period: int, rcp: str, ssp: str, horizon_year: int, tide_on: bool,
+                  storm_tide_offset_h: float, domain_area_km2: float | None = None) -> Forcing:
+    """A scenario's forcing with hydrointel's signature (engine.py:53-70), through the registered
+    resolver (``his_amd.scenarios.register_resolver``; the default is this package's design storm).
+    ``cfg`` and the domain area belong to hydrointel's own storm construction, which a resolver
+    installed from it applies; the default storm does not use them."""
+    from .scenarios import Scenario, resolve
+    return resolve(Scenario(int(return_period), rcp, ssp, int(horizon_year), bool(tide_on),
+                            float(storm_tide_offset_h)))
+
+
+def scenario_landuse(domain, ssp, horizon_year: int, base_year: int = 2025) -> np.ndarray:
+    """The domain's land use at the horizon year under an SSP (an ``SSPState`` or its name), with
+    hydrointel's signature (engine.py:73-75); the growth rule is ``api.grown_landuse``'s."""
+    from .api import grown_landuse
+    return grown_landuse(domain, ssp, int(horizon_year), base_year)
+
+
 def normal_depth(one: SWE1D, Q, slope_min: float = 1e-4) -> torch.Tensor:
     """Manning normal depth for discharge Q in every cell, by bisection (engine.py:78-101)."""
     sec = one.sec

@@ -71,6 +71,31 @@ def sea_level_rise(rcp: str, year: int) -> float:
     return at2050 + (at2100 - at2050) * (min(year, 2100) - 2050) / 50.0
 
 
+def slr_m(rcp: str, year: int) -> float:
+    """hydrointel's name for ``sea_level_rise`` (forcing/scenarios.py:48-57), which, like it, refuses a
+    year beyond the projection table."""
+    if rcp not in RCP_TABLE:
+        raise ValueError(f"unknown RCP {rcp!r} (expected one of {sorted(RCP_TABLE)})")
+    if year > 2100:
+        raise ValueError(f"horizon_year {year} beyond 2100 is outside the projection table")
+    return sea_level_rise(rcp, year)
+
+
+@dataclass(frozen=True)
+class SSPState:
+    """A pathway's socioeconomic modifiers, with hydrointel's field names (forcing/scenarios.py:69-82)."""
+    ssp: str
+    urban_expansion_rate: float
+    population_factor: float
+    drainage_investment_factor: float
+
+
+def ssp_state(ssp: str) -> SSPState:
+    if ssp not in SSP_TABLE:
+        raise ValueError(f"unknown SSP {ssp!r} (expected one of {sorted(SSP_TABLE)})")
+    return SSPState(ssp, *SSP_TABLE[ssp])
+
+
 def drainage_factor(ssp: str) -> float:
     """Multiplier on channel conveyance from the pathway's drainage investment."""
     return SSP_TABLE[ssp][2]

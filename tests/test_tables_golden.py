@@ -31,9 +31,12 @@ def test_pathways_match():
         assert S.rain_factor(rcp) == v, rcp
     for rcp, years in REF["slr_m"].items():
         for y, v in years.items():
-            assert S.sea_level_rise(rcp, int(y)) == v, (rcp, y)
+            assert S.sea_level_rise(rcp, int(y)) == v and S.slr_m(rcp, int(y)) == v, (rcp, y)
     for ssp, (urban, pop, drain) in REF["ssp"].items():
         assert S.SSP_TABLE[ssp][0] == urban and S.population_factor(ssp) == pop and S.drainage_factor(ssp) == drain
+        st = S.ssp_state(ssp)
+        assert (st.ssp, st.urban_expansion_rate, st.population_factor, st.drainage_investment_factor) == \
+            (ssp, urban, pop, drain)
 
 
 def _close(got, want, where):
