@@ -428,8 +428,7 @@ struct his_engine {
     int ex_one;               // every 1-D cell has at most one link: the exchange is one launch
     DtArgs* head_dev;         // device copy of D for the fused head
     int32_t* dtg_of;          // (Btot) device: dt group of each member
-    hipGraphExec_t exec[2];   // [0]: one step, [1]: kmulti steps
-    int kmulti;
+    hipGraphExec_t exec[3];   // one captured graph per entry of kGraphSteps
     int graphs_ready;
     void* step_max[HIS_MAX_GROUPS];   // per grid group: running maximum of h folded every step, or NULL
 };
@@ -831,10 +830,14 @@ extern "C" int his_engine_refresh_members(his_engine* e, const int32_t* mask, vo
 
 static void engine_drop_graphs(his_engine* e);
 
+// Steps per captured graph.  Each graph ends with a standalone ledger reduction and its launch
+// meets the queue empty, which costs about 20 us, so long runs replay the longest graph; the shorter
+// ones take the remainder.
+static constexpr int kGraphSteps[3] = {1, 8, 32};
+
 static int engine_build_graphs(his_engine* e) {
-    e->kmulti = 8;
-    for (int k = 0; k < 2; ++k) {
-        const int n = k == 0 ? 1 : e->kmulti;
+    for (int k = 0; k < 3; ++k) {
+        const int n = kGraphSteps[k];
         hipGraph_t g = nullptr;
         HIS_HIP(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));
         int rc = 0;
@@ -849,7 +852,7 @@ static int engine_build_graphs(his_engine* e) {
         }
         if (g) (void)hipGraphDestroy(g);
         if (rc) {
-            if (k == 1) (void)hipGraphExecDestroy(e->exec[0]);   // the one-step graph made above
+            for (int j = 0; j < k; ++j) (void)hipGraphExecDestroy(e->exec[j]);   // the graphs made above
             return rc;
         }
     }
@@ -896,11 +899,12 @@ extern "C" int his_engine_step(his_engine* e, int32_t n_steps, void* stream) {
         if (rc) return rc;
     }
     int left = n_steps;
-    while (left >= e->kmulti) {
-        HIS_HIP(hipGraphLaunch(e->exec[1], s));
-        left -= e->kmulti;
+    for (int k = 2; k >= 0; --k) {
+        while (left >= kGraphSteps[k]) {
+            HIS_HIP(hipGraphLaunch(e->exec[k], s));
+            left -= kGraphSteps[k];
+        }
     }
-    while (left-- > 0) HIS_HIP(hipGraphLaunch(e->exec[0], s));
     return 0;
 }
 
@@ -915,8 +919,7 @@ extern "C" int his_engine_fold_max(his_engine* e, int32_t group, void* hmax, voi
 static void engine_drop_graphs(his_engine* e) {
     if (!e->graphs_ready) return;
     (void)hipDeviceSynchronize();   // a replay may still be running on the caller's stream
-    (void)hipGraphExecDestroy(e->exec[0]);
-    (void)hipGraphExecDestroy(e->exec[1]);
+    for (int k = 0; k < 3; ++k) (void)hipGraphExecDestroy(e->exec[k]);
     e->graphs_ready = 0;
 }
 
