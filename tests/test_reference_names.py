@@ -8,7 +8,9 @@
 * ``engine.build_forcing`` with hydrointel's signature is the registered resolver's storm for the same
   scenario, ``engine.BatchForcing`` stacks members like ``Forcing.stack``, and
   ``engine.scenario_landuse`` takes an ``SSPState`` as hydrointel's does;
-* ``scenarios.slr_m`` refuses a year beyond the projection table, as hydrointel's does.
+* ``scenarios.slr_m`` refuses a year beyond the projection table, as hydrointel's does;
+* ``EdgeBC`` refuses unknown kinds and value-less stage or discharge edges, and gives per-member
+  values as the callable returns them; ``all_edges`` and ``edge_values`` keep the edge order.
 """
 import numpy as np
 import pytest
@@ -70,3 +72,18 @@ def test_sea_level_rise_refuses_years_past_the_table():
         S.slr_m("RCP4.5", 2101)
     with pytest.raises(ValueError):
         S.slr_m("RCP7.0", 2050)
+
+
+def test_edge_conditions():
+    from his_amd.solver.boundaries import EDGES, EdgeBC, all_edges, edge_values
+    with pytest.raises(ValueError):
+        EdgeBC("wall")
+    with pytest.raises(ValueError):
+        EdgeBC("stage")
+    tide = EdgeBC("stage", lambda t: 0.5 + t)
+    assert tide.at(2.0) == 2.5 and isinstance(tide.at(2.0), float)
+    per_member = EdgeBC("discharge", lambda t: np.array([1.0, 2.0]) * t)
+    assert np.array_equal(per_member.at(3.0), [3.0, 6.0])
+    bcs = all_edges("closed", east=EdgeBC("stage", 1.25), south="transmissive")
+    assert [bcs[e].kind for e in EDGES] == ["closed", "stage", "transmissive", "closed"]
+    assert edge_values(bcs, 0.0) == [0.0, 1.25, 0.0, 0.0]
