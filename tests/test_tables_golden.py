@@ -1,7 +1,7 @@
 """Land-use table, climate and socioeconomic pathways and agreement criteria against the values
 the reference's own functions returned (tests/golden/make_tables_golden.py): the eight classes'
 parameters, the rain factor and sea-level rise of every RCP at every horizon year from 2010 to
-2100, the SSP modifiers, and every metric, limit and verdict of ``agreement`` on five seeded pairs
+2100, the SSP modifiers, and every metric, limit and pass/fail result of ``agreement`` on five seeded pairs
 of runs (passing, failing, no flooded area in the first run, no deep cell).  Equal to the last bit
 except the agreement metrics, which are sums and percentiles over arrays and are held to 1e-12
 relative."""
