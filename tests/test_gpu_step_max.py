@@ -1,8 +1,8 @@
 """The every-step depth maximum (``his_engine_track_step_max``).
 
 Folded after the 2-D stages of every coupled step, it must equal the maximum of the depth taken on
-the host after each single step; an engine stepping through the 8-step graph must reach the same
-maximum and state; and stopping it must leave the maximum as it was.  The maximum is exact
+the host after each single step; an engine stepping through the 32-step, 8-step and single-step
+graphs must reach the same maximum and state; and stopping it must leave the maximum as it was.  The maximum is exact
 arithmetic, so the comparisons are bit for bit.
 """
 import pytest
@@ -30,19 +30,19 @@ def test_step_max_is_the_max_over_every_step(gpu, precision):
     hmax = torch.zeros_like(two.h)
     dev = _start(eng, hmax)
     ref = torch.zeros_like(two.h)
-    for _ in range(12):
+    for _ in range(44):
         dev.step(1)
         torch.maximum(ref, two.h, out=ref)
     torch.cuda.synchronize()
     assert bool((ref > 0).any())
     assert torch.equal(hmax, ref)
 
-    # the 8-step graph and then four single steps: the same maximum and the same state
+    # 44 steps in one call: the 32-step graph, the 8-step graph and four single steps; the same
+    # maximum and the same state
     eng2, _ = coupled_case(gpu, precision)
     hmax2 = torch.zeros_like(eng2.two.h)
     dev2 = _start(eng2, hmax2)
-    dev2.step(8)
-    dev2.step(4)
+    dev2.step(44)
     torch.cuda.synchronize()
     assert torch.equal(eng2.two.h, two.h)
     assert torch.equal(hmax2, hmax)
