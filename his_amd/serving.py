@@ -288,11 +288,12 @@ class Server:
     """Slot pools for the serving grid sizes in one device engine, fed by a request stream."""
 
     def __init__(self, slots=None, precision: str = "fp32bulk", hbm_budget_bytes: float | None = None,
-                 device=None, grids=SERVING_GRIDS, chunk: int = 16, keep_fields: bool = False, tune: bool = True,
+                 device=None, grids=SERVING_GRIDS, chunk: int = 32, keep_fields: bool = False, tune: bool = True,
                  max_slots=(24, 16, 16)):
         """``hbm_budget_bytes`` (default: a quarter of the device's memory) bounds the HBM that busy
         slots may hold; a request is admitted only while it fits.  ``slots`` per grid size default to
-        what an equal share of the budget holds, at most ``max_slots``."""
+        what an equal share of the budget holds, at most ``max_slots``.  ``chunk`` coupled steps run
+        between two looks at the clocks (one 32-step graph replay by default)."""
         from .batching import GroupedEngine
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         self.cfg = SolverConfig().replace(precision=precision)
